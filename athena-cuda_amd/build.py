@@ -1,0 +1,62 @@
+"""Build the HIP library of the MHD step for gfx950 (in-tree, no JIT cache).
+
+    python athena-cuda_amd/build.py          # -> athena-cuda_amd/libathmi.so
+
+The step kernels are compiled twice from one source: the "exact" variant with
+-ffp-contract=off (one rounding per operation, as the reference CPU build)
+and the "fast" variant with FMA contraction. Host code that must reproduce
+the reference's host arithmetic (problem generators) is built by g++ with
+-ffp-contract=off; the runtime is built by hipcc.
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INC = os.path.join(ROOT, "include")
+OUT = os.path.join(PKG, "libathmi.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "--offload-arch=gfx950"
+# packed FP32 instructions stay off on this pool (they have given wrong results)
+NO_PK = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-I", INC, "-I", CSRC]
+
+
+def run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose_resources=False):
+    with tempfile.TemporaryDirectory(prefix="athmi_build_") as tmp:
+        objs = []
+        for variant, contract in (("exact", "off"), ("fast", "fast")):
+            o = os.path.join(tmp, f"step_{variant}.o")
+            cmd = [HIPCC, ARCH, *COMMON, *NO_PK, f"-DATHMI_VARIANT={variant}",
+                   f"-ffp-contract={contract}", "-c", os.path.join(CSRC, "step_kernels.hip"),
+                   "-o", o]
+            if verbose_resources:
+                cmd.append("-Rpass-analysis=kernel-resource-usage")
+            run(cmd)
+            objs.append(o)
+        o = os.path.join(tmp, "athmi_capi.o")
+        run([HIPCC, ARCH, *COMMON, *NO_PK, "-ffp-contract=off", "-c",
+             os.path.join(CSRC, "athmi_capi.cpp"), "-o", o])
+        objs.append(o)
+        for src in ("host_problems.cpp", "dropin.cpp"):
+            o = os.path.join(tmp, src.replace(".cpp", ".o"))
+            run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I", INC, "-I", CSRC,
+                 "-c", os.path.join(CSRC, src), "-o", o])
+            objs.append(o)
+        tmp_out = OUT + ".tmp"
+        run([HIPCC, ARCH, "-shared", "-fPIC", "-o", tmp_out, *objs])
+        os.replace(tmp_out, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose_resources="-v" in sys.argv)

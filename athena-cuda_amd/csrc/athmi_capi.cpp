@@ -1,0 +1,539 @@
+// athmi_capi.cpp -- host runtime behind include/athmi.h: device memory of the
+// instances (SoA planes in HBM, ping-pong state buffers, scratch planes and a
+// device-resident clock per instance), the tile work-list of a batch, HIP
+// graph capture of one main-loop step, and the AoS<->SoA host transfers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "athmi.h"
+#include "host_problems.hpp"
+#include "step_kernels.hpp"
+
+ATHMI_DECLARE_VARIANT(exact)
+ATHMI_DECLARE_VARIANT(fast)
+
+using namespace athmi;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(ATHMI_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+struct Ops {
+  hipError_t (*integrate)(const void*, hipStream_t);
+  hipError_t (*begin)(const void*, int, hipStream_t);
+  hipError_t (*end)(const void*, int, hipStream_t);
+  hipError_t (*ghost)(const void*, int, int, hipStream_t);
+  hipError_t (*new_dt)(const void*, hipStream_t);
+  hipError_t (*divb)(const void*, hipStream_t);
+};
+
+const Ops kExact = {athmi_exact_launch_integrate, athmi_exact_launch_begin,
+                    athmi_exact_launch_end,       athmi_exact_launch_ghost,
+                    athmi_exact_launch_new_dt,    athmi_exact_launch_divb};
+const Ops kFast = {athmi_fast_launch_integrate, athmi_fast_launch_begin,
+                   athmi_fast_launch_end,       athmi_fast_launch_ghost,
+                   athmi_fast_launch_new_dt,    athmi_fast_launch_divb};
+
+size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
+
+}  // namespace
+
+struct athmi_grid {
+  int n = 0;
+  int variant = ATHMI_EXACT;
+  std::vector<athmi_desc> desc;
+  std::vector<Inst> hinst;
+  Inst* dinst = nullptr;
+  Clock* dclk = nullptr;
+  double* dmem = nullptr;
+  size_t dmem_bytes = 0;
+  BatchLaunch bl{};
+  Ops ops{};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  bool use_graph = true;
+  hipGraphExec_t graph = nullptr;
+  std::vector<std::vector<double>> cpaw_soln;  // initial cpaw2d states (8 planes)
+  std::vector<double> stage;                   // host SoA staging
+
+  void drop_graph() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+  }
+  int refresh_tiles() {
+    int tot[NKINDS] = {0};
+    for (int k = 0; k < n; k++) {
+      Inst& I = hinst[k];
+      for (int kind = 0; kind < NKINDS; kind++) {
+        I.tile0[kind] = tot[kind];
+        tot[kind] += tiles_of(kind, I.nx1, I.nx2, bl.shape.x2_seg);
+      }
+    }
+    for (int kind = 0; kind < NKINDS; kind++) bl.ntiles[kind] = tot[kind];
+    HIP_TRY(hipMemcpy(dinst, hinst.data(), sizeof(Inst) * n, hipMemcpyHostToDevice));
+    drop_graph();
+    return ATHMI_OK;
+  }
+  int read_clock(int k, Clock& c) {
+    HIP_TRY(hipStreamSynchronize(stream));
+    HIP_TRY(hipMemcpy(&c, dclk + k, offsetof(Clock, divb_bits) + sizeof(unsigned long long),
+                      hipMemcpyDeviceToHost));
+    return ATHMI_OK;
+  }
+  int write_clock_head(int k, const Clock& c) {
+    HIP_TRY(hipMemcpy(dclk + k, &c, offsetof(Clock, divb_bits), hipMemcpyHostToDevice));
+    return ATHMI_OK;
+  }
+};
+
+extern "C" {
+
+const char* athmi_last_error(void) { return g_err.c_str(); }
+
+int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
+  if (!desc || !out || n <= 0) return fail(ATHMI_EINVAL, "athmi_create: bad arguments");
+  for (int k = 0; k < n; k++) {
+    const athmi_desc& d = desc[k];
+    if (d.nx1 < 2 || d.nx2 < 2)
+      return fail(ATHMI_EINVAL, "athmi_create: the 2-D integrator needs Nx1 > 1 and Nx2 > 1");
+    if (!(d.x1max > d.x1min) || !(d.x2max > d.x2min))
+      return fail(ATHMI_EINVAL, "athmi_create: empty domain");
+    const int bcs[4] = {d.bc_ix1, d.bc_ox1, d.bc_ix2, d.bc_ox2};
+    for (int b : bcs)
+      if (b != ATHMI_BC_REFLECT && b != ATHMI_BC_OUTFLOW && b != ATHMI_BC_PERIODIC)
+        return fail(ATHMI_EINVAL, "athmi_create: boundary flag must be 1, 2 or 4");
+    if (d.flux != ATHMI_FLUX_ROE && d.flux != ATHMI_FLUX_HLLE)
+      return fail(ATHMI_EINVAL, "athmi_create: unknown flux");
+    if (d.variant != desc[0].variant)
+      return fail(ATHMI_EINVAL, "athmi_create: one arithmetic variant per handle");
+  }
+  if (desc[0].variant != ATHMI_EXACT && desc[0].variant != ATHMI_FAST)
+    return fail(ATHMI_EINVAL, "athmi_create: unknown variant");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+
+  athmi_grid* g = new athmi_grid();
+  g->n = n;
+  g->variant = desc[0].variant;
+  g->ops = (g->variant == ATHMI_FAST) ? kFast : kExact;
+  g->desc.assign(desc, desc + n);
+  g->hinst.resize(n);
+  g->cpaw_soln.resize(n);
+  g->bl.ninst = n;
+  g->bl.ncu = prop.multiProcessorCount;
+  g->bl.shape.x2_seg = 32;
+  g->bl.shape.persist = 0;
+  // device memory: per instance 2*NSTATE state planes + NSCRATCH scratch planes
+  std::vector<size_t> off(n);
+  size_t total = 0;
+  for (int k = 0; k < n; k++) {
+    const athmi_desc& d = desc[k];
+    Inst& I = g->hinst[k];
+    std::memset(&I, 0, sizeof(I));
+    I.nx1 = d.nx1;
+    I.nx2 = d.nx2;
+    I.is = 4;
+    I.ie = d.nx1 + 3;
+    I.js = 4;
+    I.je = d.nx2 + 3;
+    I.sx = d.nx1 + 8;
+    I.sy = d.nx2 + 8;
+    I.pitch = (int)round_up(I.sx, 16);
+    I.flux = d.flux;
+    I.bc[0] = d.bc_ix1;
+    I.bc[1] = d.bc_ox1;
+    I.bc[2] = d.bc_ix2;
+    I.bc[3] = d.bc_ox2;
+    I.dx1 = (d.x1max - d.x1min) / (double)d.nx1;  // init_grid.c:60
+    I.dx2 = (d.x2max - d.x2min) / (double)d.nx2;
+    I.gamma = d.gamma;
+    I.g1 = d.gamma - 1.0;  // main.c:93-94
+    I.g2 = d.gamma - 2.0;
+    I.ps = round_up((size_t)I.pitch * I.sy, 64);
+    off[k] = total;
+    total += (size_t)(2 * NSTATE + NSCRATCH) * I.ps;
+  }
+  g->dmem_bytes = total * sizeof(double);
+  if (hipMalloc(&g->dmem, g->dmem_bytes) != hipSuccess) {
+    delete g;
+    return fail(ATHMI_ENOMEM, "athmi_create: hipMalloc of the state failed");
+  }
+  (void)hipMemset(g->dmem, 0, g->dmem_bytes);
+  if (hipMalloc(&g->dclk, sizeof(Clock) * n) != hipSuccess ||
+      hipMalloc(&g->dinst, sizeof(Inst) * n) != hipSuccess) {
+    athmi_destroy(g);
+    return fail(ATHMI_ENOMEM, "athmi_create: hipMalloc of the descriptors failed");
+  }
+  std::vector<Clock> hc(n);
+  for (int k = 0; k < n; k++) {
+    std::memset(&hc[k], 0, sizeof(Clock));
+    hc[k].cour_no = desc[k].cour_no;
+    hc[k].tlim = HUGE_VAL;
+    hc[k].nlim = -1;
+    Inst& I = g->hinst[k];
+    I.state[0] = g->dmem + off[k];
+    I.state[1] = I.state[0] + NSTATE * I.ps;
+    I.scratch = I.state[1] + NSTATE * I.ps;
+    I.clk = g->dclk + k;
+  }
+  (void)hipMemcpy(g->dclk, hc.data(), sizeof(Clock) * n, hipMemcpyHostToDevice);
+  if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
+    athmi_destroy(g);
+    return fail(ATHMI_EHIP, "athmi_create: stream creation failed");
+  }
+  g->own_stream = true;
+  g->bl.insts = g->dinst;
+  int rc = g->refresh_tiles();
+  if (rc != ATHMI_OK) {
+    athmi_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return ATHMI_OK;
+}
+
+int athmi_destroy(athmi_grid* g) {
+  if (!g) return ATHMI_OK;
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  g->drop_graph();
+  if (g->dmem) (void)hipFree(g->dmem);
+  if (g->dclk) (void)hipFree(g->dclk);
+  if (g->dinst) (void)hipFree(g->dinst);
+  if (g->own_stream && g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+  return ATHMI_OK;
+}
+
+int athmi_count(const athmi_grid* g, int* n) {
+  if (!g || !n) return fail(ATHMI_EINVAL, "athmi_count: null argument");
+  *n = g->n;
+  return ATHMI_OK;
+}
+
+static int check_inst(const athmi_grid* g, int inst) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  if (inst < 0 || inst >= g->n) return fail(ATHMI_EINVAL, "instance index out of range");
+  return ATHMI_OK;
+}
+
+int athmi_set_spacing(athmi_grid* g, int inst, double dx1, double dx2) {
+  int rc = check_inst(g, inst);
+  if (rc) return rc;
+  if (!(dx1 > 0.0) || !(dx2 > 0.0)) return fail(ATHMI_EINVAL, "athmi_set_spacing: dx <= 0");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->hinst[inst].dx1 = dx1;
+  g->hinst[inst].dx2 = dx2;
+  return g->refresh_tiles();
+}
+
+int athmi_upload(athmi_grid* g, int inst, const double* gas, const double* b1i, const double* b2i,
+                 const double* b3i) {
+  int rc = check_inst(g, inst);
+  if (rc) return rc;
+  if (!gas || !b1i || !b2i || !b3i) return fail(ATHMI_EINVAL, "athmi_upload: null array");
+  Clock c;
+  if ((rc = g->read_clock(inst, c))) return rc;
+  const Inst& I = g->hinst[inst];
+  g->stage.assign((size_t)NSTATE * I.ps, 0.0);
+  double* s = g->stage.data();
+  for (int j = 0; j < I.sy; j++)
+    for (int i = 0; i < I.sx; i++) {
+      const size_t h = (size_t)j * I.sx + i, k = (size_t)j * I.pitch + i;
+      for (int q = 0; q < 8; q++) s[q * I.ps + k] = gas[8 * h + q];
+      s[F_B1I * I.ps + k] = b1i[h];
+      s[F_B2I * I.ps + k] = b2i[h];
+      s[F_B3I * I.ps + k] = b3i[h];
+    }
+  HIP_TRY(hipMemcpy(I.state[c.cur], s, sizeof(double) * NSTATE * I.ps, hipMemcpyHostToDevice));
+  return ATHMI_OK;
+}
+
+int athmi_download(athmi_grid* g, int inst, double* gas, double* b1i, double* b2i, double* b3i) {
+  int rc = check_inst(g, inst);
+  if (rc) return rc;
+  if (!gas || !b1i || !b2i || !b3i) return fail(ATHMI_EINVAL, "athmi_download: null array");
+  Clock c;
+  if ((rc = g->read_clock(inst, c))) return rc;
+  const Inst& I = g->hinst[inst];
+  g->stage.resize((size_t)NSTATE * I.ps);
+  double* s = g->stage.data();
+  HIP_TRY(hipMemcpy(s, I.state[c.cur], sizeof(double) * NSTATE * I.ps, hipMemcpyDeviceToHost));
+  for (int j = 0; j < I.sy; j++)
+    for (int i = 0; i < I.sx; i++) {
+      const size_t h = (size_t)j * I.sx + i, k = (size_t)j * I.pitch + i;
+      for (int q = 0; q < 8; q++) gas[8 * h + q] = s[q * I.ps + k];
+      b1i[h] = s[F_B1I * I.ps + k];
+      b2i[h] = s[F_B2I * I.ps + k];
+      b3i[h] = s[F_B3I * I.ps + k];
+    }
+  return ATHMI_OK;
+}
+
+int athmi_set_clock(athmi_grid* g, int inst, double dt, double time, int nstep) {
+  int rc = check_inst(g, inst);
+  if (rc) return rc;
+  Clock c;
+  if ((rc = g->read_clock(inst, c))) return rc;
+  c.dt = dt;
+  c.time = time;
+  c.nstep = nstep;
+  return g->write_clock_head(inst, c);
+}
+
+int athmi_get_clock(athmi_grid* g, int inst, double* dt, double* time, int* nstep) {
+  int rc = check_inst(g, inst);
+  if (rc) return rc;
+  Clock c;
+  if ((rc = g->read_clock(inst, c))) return rc;
+  if (dt) *dt = c.dt;
+  if (time) *time = c.time;
+  if (nstep) *nstep = c.nstep;
+  return ATHMI_OK;
+}
+
+int athmi_set_limits(athmi_grid* g, int inst, double tlim, int nlim) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  const int lo = inst < 0 ? 0 : inst, hi = inst < 0 ? g->n - 1 : inst;
+  if (inst >= g->n) return fail(ATHMI_EINVAL, "instance index out of range");
+  for (int k = lo; k <= hi; k++) {
+    Clock c;
+    int rc = g->read_clock(k, c);
+    if (rc) return rc;
+    c.tlim = tlim;
+    c.nlim = nlim;
+    if ((rc = g->write_clock_head(k, c))) return rc;
+  }
+  return ATHMI_OK;
+}
+
+int athmi_set_bvals(athmi_grid* g) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  HIP_TRY(g->ops.ghost(&g->bl, 0, 0, g->stream));
+  return ATHMI_OK;
+}
+
+int athmi_new_dt(athmi_grid* g) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  HIP_TRY(g->ops.new_dt(&g->bl, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return ATHMI_OK;
+}
+
+int athmi_integrate(athmi_grid* g) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  HIP_TRY(g->ops.begin(&g->bl, 1, g->stream));
+  HIP_TRY(g->ops.integrate(&g->bl, g->stream));
+  // integrate_2d writes only the interior; the ghost zones of the produced
+  // buffer carry the current values, as an in-place update would leave them
+  HIP_TRY(g->ops.ghost(&g->bl, 2, 0, g->stream));
+  HIP_TRY(g->ops.end(&g->bl, 1, g->stream));
+  return ATHMI_OK;
+}
+
+static int issue_step(athmi_grid* g) {
+  HIP_TRY(g->ops.begin(&g->bl, 0, g->stream));
+  HIP_TRY(g->ops.integrate(&g->bl, g->stream));
+  HIP_TRY(g->ops.ghost(&g->bl, 1, 1, g->stream));
+  HIP_TRY(g->ops.end(&g->bl, 0, g->stream));
+  return ATHMI_OK;
+}
+
+int athmi_step(athmi_grid* g) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  if (!g->use_graph) return issue_step(g);
+  if (!g->graph) {
+    hipGraph_t graph;
+    HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    int rc = issue_step(g);
+    hipError_t e = hipStreamEndCapture(g->stream, &graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(ATHMI_EHIP, std::string("capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&g->graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      g->graph = nullptr;
+      return fail(ATHMI_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+  }
+  HIP_TRY(hipGraphLaunch(g->graph, g->stream));
+  return ATHMI_OK;
+}
+
+int athmi_run(athmi_grid* g, int max_steps, int* steps) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  int issued = 0;
+  const int chunk = 16;
+  std::vector<Clock> hc(g->n);
+  while (max_steps < 0 || issued < max_steps) {
+    // any instance still running?
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    HIP_TRY(hipMemcpy2D(hc.data(), sizeof(Clock), g->dclk, sizeof(Clock),
+                        offsetof(Clock, steps_run), g->n, hipMemcpyDeviceToHost));
+    bool running = false;
+    for (const Clock& c : hc)
+      if (c.time < c.tlim && (c.nlim < 0 || c.nstep < c.nlim)) running = true;
+    if (!running) break;
+    int m = chunk;
+    if (max_steps >= 0) m = std::min(m, max_steps - issued);
+    for (int s = 0; s < m; s++) {
+      int rc = athmi_step(g);
+      if (rc) return rc;
+    }
+    issued += m;
+  }
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  if (steps) *steps = issued;
+  return ATHMI_OK;
+}
+
+int athmi_divb_max(athmi_grid* g, int inst, double* out) {
+  int rc = check_inst(g, inst);
+  if (rc) return rc;
+  HIP_TRY(g->ops.divb(&g->bl, g->stream));
+  Clock c;
+  if ((rc = g->read_clock(inst, c))) return rc;
+  unsigned long long bits = c.divb_bits;
+  std::memcpy(out, &bits, sizeof(double));
+  return ATHMI_OK;
+}
+
+int athmi_set_stream(athmi_grid* g, void* s) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  if (g->own_stream) (void)hipStreamDestroy(g->stream);
+  if (s) {
+    g->stream = static_cast<hipStream_t>(s);
+    g->own_stream = false;
+  } else {
+    HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    g->own_stream = true;
+  }
+  g->drop_graph();
+  return ATHMI_OK;
+}
+
+int athmi_sync(athmi_grid* g) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return ATHMI_OK;
+}
+
+int athmi_set_shape(athmi_grid* g, int x2_seg, int persist) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  if (x2_seg < 1 || x2_seg > 4096 || persist < 0 || persist > 64)
+    return fail(ATHMI_EINVAL, "athmi_set_shape: x2_seg in [1,4096], persist in [0,64]");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->bl.shape.x2_seg = x2_seg;
+  g->bl.shape.persist = persist;
+  return g->refresh_tiles();
+}
+
+int athmi_use_graph(athmi_grid* g, int on) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  g->use_graph = on != 0;
+  return ATHMI_OK;
+}
+
+int athmi_device_state(athmi_grid* g, int inst, double** planes, int* pitch) {
+  int rc = check_inst(g, inst);
+  if (rc) return rc;
+  Clock c;
+  if ((rc = g->read_clock(inst, c))) return rc;
+  const Inst& I = g->hinst[inst];
+  for (int q = 0; q < NSTATE; q++) planes[q] = I.state[c.cur] + q * I.ps;
+  if (pitch) *pitch = I.pitch;
+  return ATHMI_OK;
+}
+
+// ---- problem generators ------------------------------------------------------
+static HostGeom geom_of(const athmi_grid* g, int inst) {
+  const athmi_desc& d = g->desc[inst];
+  const Inst& I = g->hinst[inst];
+  HostGeom G;
+  G.Nx1 = d.nx1;
+  G.Nx2 = d.nx2;
+  G.is = I.is;
+  G.ie = I.ie;
+  G.js = I.js;
+  G.je = I.je;
+  G.sx = I.sx;
+  G.sy = I.sy;
+  G.dx1 = I.dx1;
+  G.dx2 = I.dx2;
+  G.x1_0 = d.x1min;
+  G.x2_0 = d.x2min;
+  G.idisp = -I.is;
+  G.jdisp = -I.js;
+  G.gamma_1 = I.g1;
+  return G;
+}
+
+static int upload_fresh(athmi_grid* g, int inst, const HostState& S) {
+  return athmi_upload(g, inst, S.gas.data(), S.b1i.data(), S.b2i.data(), S.b3i.data());
+}
+
+int athmi_pgen_blast(athmi_grid* g, int inst, double radius, double pamb, double prat, double b0,
+                     double angle_deg, double cx, double cy) {
+  int rc = check_inst(g, inst);
+  if (rc) return rc;
+  const HostGeom G = geom_of(g, inst);
+  HostState S(G);
+  problem_blast(G, S, radius, pamb, prat, b0, angle_deg, cx, cy);
+  return upload_fresh(g, inst, S);
+}
+
+int athmi_pgen_field_loop(athmi_grid* g, int inst, double rad, double amp, double vflow,
+                          int iprob) {
+  int rc = check_inst(g, inst);
+  if (rc) return rc;
+  const HostGeom G = geom_of(g, inst);
+  HostState S(G);
+  problem_field_loop(G, S, rad, amp, vflow, iprob);
+  return upload_fresh(g, inst, S);
+}
+
+int athmi_pgen_cpaw2d(athmi_grid* g, int inst, double angle, double b_par, double b_perp,
+                      double pres, double v_par, double* lambda) {
+  int rc = check_inst(g, inst);
+  if (rc) return rc;
+  const HostGeom G = geom_of(g, inst);
+  HostState S(G);
+  const double lam = problem_cpaw2d(G, S, angle, b_par, b_perp, pres, v_par, &g->cpaw_soln[inst]);
+  if (lambda) *lambda = lam;
+  return upload_fresh(g, inst, S);
+}
+
+int athmi_cpaw2d_l1(athmi_grid* g, int inst, double* out) {
+  int rc = check_inst(g, inst);
+  if (rc) return rc;
+  if (g->cpaw_soln[inst].empty())
+    return fail(ATHMI_ESTATE, "athmi_cpaw2d_l1: instance was not set up by athmi_pgen_cpaw2d");
+  const HostGeom G = geom_of(g, inst);
+  HostState S(G);
+  if ((rc = athmi_download(g, inst, S.gas.data(), S.b1i.data(), S.b2i.data(), S.b3i.data())))
+    return rc;
+  cpaw2d_errors(G, S, g->cpaw_soln[inst], out);
+  return ATHMI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,769 @@
+// step_kernels.hip -- the CTU+CT step (__integrate_2d__.c:100-537), new_dt
+// (new_dt.c:25-103) and set_bvals (set_bvals.c:114-1031) as HIP kernels for
+// gfx950 over a tile work-list spanning one or many grid instances.
+//
+// Pipeline of one step (all state in SoA planes, ping-pong buffers):
+//   k_begin      clock: limits, tlim clamp of dt, reset of dt partials
+//   k_pred_x1    Steps 1a-1e: x1 sweep, PLM via an LDS neighbour exchange
+//   k_pred_x2    Steps 2a-2e: x2 sweep, each lane marches down a column
+//   k_corner<P>  Step 4 corner emf (and Step 3 emf3_cc comes from k_pred_x1)
+//   k_corr       Steps 4 (half-step face B), 5, 6, 7, 8b, 8c
+//   k_corner<C>  Step 9 corner emf
+//   k_final      Steps 9 (face B), 11a/11b, 13 and the new_dt max, fused
+//   k_ghost      set_bvals on the new state
+//   k_end        nstep++, time += dt, dt = min(2 dt, CourNo/max_dti), swap
+//
+// This file is compiled twice: ATHMI_VARIANT=exact with -ffp-contract=off
+// (one rounding per operation, as the reference CPU build) and
+// ATHMI_VARIANT=fast with FMA contraction.
+#include <hip/hip_runtime.h>
+
+#include "mhd_device.hpp"
+#include "step_kernels.hpp"
+
+#ifndef ATHMI_VARIANT
+#error "ATHMI_VARIANT must name the arithmetic variant"
+#endif
+
+namespace athmi {
+namespace ATHMI_VARIANT {
+
+__device__ __forceinline__ int dcdiv(int a, int b) { return (a + b - 1) / b; }
+
+// instance owning global tile t of kernel kind `kind` (tile0 ascending)
+__device__ __forceinline__ int find_inst(const Inst* __restrict__ in, int n, int kind, int t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (in[mid].tile0[kind] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ Prim7 lds_prim(double (*s)[kX1Block], int t) {
+  return Prim7{s[0][t], s[1][t], s[2][t], s[3][t], s[4][t], s[5][t], s[6][t]};
+}
+__device__ __forceinline__ void lds_put(double (*s)[kX1Block], int t, const Prim7& w) {
+  s[0][t] = w.d; s[1][t] = w.vx; s[2][t] = w.vy; s[3][t] = w.vz;
+  s[4][t] = w.p; s[5][t] = w.by; s[6][t] = w.bz;
+}
+__device__ __forceinline__ void put7(double* base, size_t ps, size_t k, const Cons7& u) {
+  base[0 * ps + k] = u.d; base[1 * ps + k] = u.mx; base[2 * ps + k] = u.my;
+  base[3 * ps + k] = u.mz; base[4 * ps + k] = u.e; base[5 * ps + k] = u.by;
+  base[6 * ps + k] = u.bz;
+}
+__device__ __forceinline__ Cons7 get7(const double* base, size_t ps, size_t k) {
+  return Cons7{base[0 * ps + k], base[1 * ps + k], base[2 * ps + k], base[3 * ps + k],
+               base[4 * ps + k], base[5 * ps + k], base[6 * ps + k]};
+}
+
+__device__ __forceinline__ double block_max256(double v, double* red) {
+  // wave-level max with cross-lane shuffles, then across the 4 waves via LDS
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = dmax(v, __shfl_xor(v, off, 64));
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  double m = red[0];
+  const int nw = (blockDim.x * blockDim.y) >> 6;
+  for (int w = 1; w < nw; w++) m = dmax(m, red[w]);
+  __syncthreads();
+  return m;
+}
+
+__device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double v) {
+  atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+}
+
+// ---------------------------------------------------------------------------
+// Steps 1a-1e (__integrate_2d__.c:128-180): one row segment per tile. Each
+// thread converts one cell to primitives; neighbours are exchanged in LDS for
+// the characteristic PLM (lr_states_plm.c:95-245); each face then gets the
+// 1/2 dt MHD source on By, prim->cons and the Riemann flux.
+__global__ void __launch_bounds__(kX1Block) k_pred_x1(const Inst* __restrict__ insts, int ninst,
+                                                      int ntiles) {
+  __shared__ double sw[7][kX1Block];
+  __shared__ double swl[7][kX1Block];
+  __shared__ double swr[7][kX1Block];
+  const int tid = threadIdx.x;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Inst& g = insts[find_inst(insts, ninst, K_PRED_X1, t)];
+    const Clock* clk = g.clk;
+    if (!clk->active) continue;
+    const int local = t - g.tile0[K_PRED_X1];
+    const int nseg = dcdiv(g.nx1 + 4, kX1Faces);
+    const int row = local / nseg, seg = local - row * nseg;
+    const int j = g.js - 2 + row;
+    const int f0 = g.is - 1 + seg * kX1Faces;
+    const int cell = f0 - 2 + tid;
+    const double dt = clk->dt;
+    const double dtodx1 = dt / g.dx1;
+    const double hdt = 0.5 * dt;
+    const EosC eos{g.gamma, g.g1, g.g2};
+    const double* __restrict__ st = g.state[clk->cur];
+    double* __restrict__ sc = g.scratch;
+    const size_t ps = g.ps;
+    const size_t rk = (size_t)j * g.pitch;
+    double bxc = 0.0;
+    if (cell >= g.is - 3 && cell <= g.ie + 3) {
+      const size_t k = rk + cell;
+      const double d = st[F_D * ps + k], m1 = st[F_M1 * ps + k], m2 = st[F_M2 * ps + k];
+      const double b1c = st[F_B1C * ps + k], b2c = st[F_B2C * ps + k];
+      const Cons7 u{d, m1, m2, st[F_M3 * ps + k], st[F_E * ps + k], b2c, st[F_B3C * ps + k]};
+      bxc = b1c;
+      lds_put(sw, tid, cons_to_prim(u, bxc, eos));
+      // Step 3 (:249-255): emf3 at t^n over [jl,ju] x [il,iu]
+      if (cell >= g.is - 2 && cell <= g.ie + 2 && tid >= 1 && tid <= kX1Block - 2)
+        sc[S_ECC * ps + k] = (b1c * m2 - b2c * m1) / d;
+    }
+    __syncthreads();
+    if (tid >= 1 && tid <= kX1Block - 2 && cell >= g.is - 2 && cell <= g.ie + 2) {
+      Prim7 wl, wr;
+      plm_cell(lds_prim(sw, tid - 1), lds_prim(sw, tid), lds_prim(sw, tid + 1), bxc, dtodx1, eos,
+               wl, wr);
+      lds_put(swl, tid + 1, wl);
+      lds_put(swr, tid, wr);
+    }
+    __syncthreads();
+    if (tid >= 2 && tid <= kX1Block - 2 && cell <= g.ie + 2) {
+      const size_t k = rk + cell;
+      Prim7 wl = lds_prim(swl, tid), wr = lds_prim(swr, tid);
+      const double bm = st[F_B1I * ps + k - 1], b0 = st[F_B1I * ps + k],
+                   bp = st[F_B1I * ps + k + 1];
+      // MHD source terms for 1/2 dt (:156-164)
+      double src = (st[F_M2 * ps + k - 1] / st[F_D * ps + k - 1]) * (b0 - bm) / g.dx1;
+      wl.by = wl.by + hdt * src;
+      src = (st[F_M2 * ps + k] / st[F_D * ps + k]) * (bp - b0) / g.dx1;
+      wr.by = wr.by + hdt * src;
+      const Cons7 ul = prim_to_cons(wl, b0, eos);
+      const Cons7 ur = prim_to_cons(wr, b0, eos);
+      put7(sc + S_UL1 * ps, ps, k, ul);
+      put7(sc + S_UR1 * ps, ps, k, ur);
+      put7(sc + S_F1 * ps, ps, k, riemann(g.flux, ul, ur, wl, wr, b0, eos));
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Steps 2a-2e (__integrate_2d__.c:189-241): the x2 sweep in the permuted
+// order (d, M2, M3, M1, E, B3c, B1c) with Bx = B2c. Lanes own adjacent
+// columns (coalesced rows); each lane marches down a segment of faces keeping
+// the three-cell PLM stencil and the pending left state in registers.
+__global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ insts, int ninst,
+                                                      int ntiles, int seg_len) {
+  const int lane = threadIdx.x;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Inst& g = insts[find_inst(insts, ninst, K_PRED_X2, t)];
+    const Clock* clk = g.clk;
+    if (!clk->active) continue;
+    const int local = t - g.tile0[K_PRED_X2];
+    const int ncb = dcdiv(g.nx1 + 5, kX2Block);
+    const int sg = local / ncb, cb = local - sg * ncb;
+    const int i = g.is - 2 + cb * kX2Block + lane;
+    if (i > g.ie + 2) continue;
+    const int jf0 = g.js - 1 + sg * seg_len;
+    const int jf1 = min(jf0 + seg_len - 1, g.je + 2);
+    const double dt = clk->dt;
+    const double dtodx2 = dt / g.dx2;
+    const double hdt = 0.5 * dt;
+    const EosC eos{g.gamma, g.g1, g.g2};
+    const double* __restrict__ st = g.state[clk->cur];
+    double* __restrict__ sc = g.scratch;
+    const size_t ps = g.ps;
+    const int pitch = g.pitch;
+    auto prim_at = [&](int jj, double& bx) -> Prim7 {
+      const size_t k = (size_t)jj * pitch + i;
+      const Cons7 u{st[F_D * ps + k], st[F_M2 * ps + k], st[F_M3 * ps + k], st[F_M1 * ps + k],
+                    st[F_E * ps + k], st[F_B3C * ps + k], st[F_B1C * ps + k]};
+      bx = st[F_B2C * ps + k];
+      return cons_to_prim(u, bx, eos);
+    };
+    double bxa, bxb, bxc;
+    Prim7 wa = prim_at(jf0 - 2, bxa);
+    Prim7 wb = prim_at(jf0 - 1, bxb);
+    Prim7 wc = prim_at(jf0, bxc);
+    Prim7 wl_cur, wr_unused;
+    plm_cell(wa, wb, wc, bxb, dtodx2, eos, wl_cur, wr_unused);
+    for (int j = jf0; j <= jf1; j++) {
+      wa = wb;
+      wb = wc;
+      bxb = bxc;
+      wc = prim_at(j + 1, bxc);
+      Prim7 wl_next, wr_cur;
+      plm_cell(wa, wb, wc, bxb, dtodx2, eos, wl_next, wr_cur);
+      const size_t k = (size_t)j * pitch + i;
+      const double bm = st[F_B2I * ps + k - pitch], b0 = st[F_B2I * ps + k],
+                   bp = st[F_B2I * ps + k + pitch];
+      // MHD source terms for 1/2 dt (:216-224)
+      double src = (st[F_M1 * ps + k - pitch] / st[F_D * ps + k - pitch]) * (b0 - bm) / g.dx2;
+      wl_cur.bz = wl_cur.bz + hdt * src;
+      src = (st[F_M1 * ps + k] / st[F_D * ps + k]) * (bp - b0) / g.dx2;
+      wr_cur.bz = wr_cur.bz + hdt * src;
+      const Cons7 ul = prim_to_cons(wl_cur, b0, eos);
+      const Cons7 ur = prim_to_cons(wr_cur, b0, eos);
+      put7(sc + S_UL2 * ps, ps, k, ul);
+      put7(sc + S_UR2 * ps, ps, k, ur);
+      put7(sc + S_F2 * ps, ps, k, riemann(g.flux, ul, ur, wl_cur, wr_cur, b0, eos));
+      wl_cur = wl_next;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// integrate_emf3_corner (__integrate_2d__.c:632-702): upwinded corner emf from
+// the face fluxes (x1-flux of B2 is -E3, x2-flux of B1 is +E3) and the
+// cell-centred emf3. PRED: Step 4 over [js-1,je+2]x[is-1,ie+2] from predictor
+// fluxes; otherwise Step 9 over [js,je+1]x[is,ie+1], the corners Step 9 uses.
+template <bool PRED>
+__global__ void __launch_bounds__(kTileX* kTileY) k_corner(const Inst* __restrict__ insts,
+                                                            int ninst, int ntiles) {
+  const int kind = PRED ? K_CORNER_P : K_CORNER_C;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Inst& g = insts[find_inst(insts, ninst, kind, t)];
+    if (!g.clk->active) continue;
+    const int local = t - g.tile0[kind];
+    const int w = PRED ? g.nx1 + 4 : g.nx1 + 2;
+    const int ntx = dcdiv(w, kTileX);
+    const int ty = local / ntx, tx = local - ty * ntx;
+    const int i = (PRED ? g.is - 1 : g.is) + tx * kTileX + threadIdx.x;
+    const int j = (PRED ? g.js - 1 : g.js) + ty * kTileY + threadIdx.y;
+    if (i > g.ie + (PRED ? 2 : 1) || j > g.je + (PRED ? 2 : 1)) continue;
+    const size_t ps = g.ps;
+    const int pitch = g.pitch;
+    double* __restrict__ sc = g.scratch;
+    const double* F1 = sc + (PRED ? S_F1 : S_F1C) * ps;
+    const double* F2 = sc + (PRED ? S_F2 : S_F2C) * ps;
+    const double* ecc = sc + (PRED ? S_ECC : S_ECH) * ps;
+    const size_t k = (size_t)j * pitch + i;
+    const size_t kjm = k - pitch, kim = k - 1, kjmim = k - pitch - 1;
+    // component offsets within a Cons1D group: d = 0, By = 5, Bz = 6
+    const double f1d_jm = F1[kjm], f1d = F1[k];
+    const double f1by_jm = F1[5 * ps + kjm], f1by = F1[5 * ps + k];
+    const double f2d_im = F2[kim], f2d = F2[k];
+    const double f2bz_im = F2[6 * ps + kim], f2bz = F2[6 * ps + k];
+    const double e_jmim = ecc[kjmim], e_jm = ecc[kjm], e_im = ecc[kim], e_00 = ecc[k];
+    double emf_l1, emf_r1, emf_l2, emf_r2;
+    if (f1d_jm > 0.0) emf_l2 = -f1by_jm + (f2bz_im - e_jmim);
+    else if (f1d_jm < 0.0) emf_l2 = -f1by_jm + (f2bz - e_jm);
+    else emf_l2 = -f1by_jm + 0.5 * (f2bz_im - e_jmim + f2bz - e_jm);
+    if (f1d > 0.0) emf_r2 = -f1by + (f2bz_im - e_im);
+    else if (f1d < 0.0) emf_r2 = -f1by + (f2bz - e_00);
+    else emf_r2 = -f1by + 0.5 * (f2bz_im - e_im + f2bz - e_00);
+    if (f2d_im > 0.0) emf_l1 = f2bz_im + (-f1by_jm - e_jmim);
+    else if (f2d_im < 0.0) emf_l1 = f2bz_im + (-f1by - e_im);
+    else emf_l1 = f2bz_im + 0.5 * (-f1by_jm - e_jmim - f1by - e_im);
+    if (f2d > 0.0) emf_r1 = f2bz + (-f1by_jm - e_jm);
+    else if (f2d < 0.0) emf_r1 = f2bz + (-f1by - e_00);
+    else emf_r1 = f2bz + 0.5 * (-f1by_jm - e_jm - f1by - e_00);
+    sc[S_EMF * ps + k] = 0.25 * (emf_l1 + emf_r1 + emf_l2 + emf_r2);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Steps 4-8 for one cell (j,i) of [js-1,je+1]x[is-1,ie+1]:
+//  - half-step face fields from the predictor corner emf (:266-275)
+//  - x1-face states: transverse x2-flux gradients + MHD sources (:284-334),
+//    then the corrector x1 flux (:438-447) for i >= is
+//  - x2-face states: transverse x1-flux gradients + MHD sources (:343-393),
+//    then the corrector x2 flux (:455-464) for j >= js
+//  - d, M1, M2 at t^{n+1/2} and the cell-centred emf3 there (:402-431)
+__global__ void __launch_bounds__(kTileX* kTileY) k_corr(const Inst* __restrict__ insts,
+                                                          int ninst, int ntiles) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Inst& g = insts[find_inst(insts, ninst, K_CORR, t)];
+    const Clock* clk = g.clk;
+    if (!clk->active) continue;
+    const int local = t - g.tile0[K_CORR];
+    const int ntx = dcdiv(g.nx1 + 3, kTileX);
+    const int ty = local / ntx, tx = local - ty * ntx;
+    const int i = g.is - 1 + tx * kTileX + threadIdx.x;
+    const int j = g.js - 1 + ty * kTileY + threadIdx.y;
+    if (i > g.ie + 1 || j > g.je + 1) continue;
+    const double dt = clk->dt;
+    const double dtodx1 = dt / g.dx1, dtodx2 = dt / g.dx2;
+    const double hdtodx1 = 0.5 * dtodx1, hdtodx2 = 0.5 * dtodx2;
+    const EosC eos{g.gamma, g.g1, g.g2};
+    const double* __restrict__ st = g.state[clk->cur];
+    double* __restrict__ sc = g.scratch;
+    const size_t ps = g.ps;
+    const int pitch = g.pitch;
+    const size_t k = (size_t)j * pitch + i;
+    const double* emf = sc + S_EMF * ps;
+    const double* B1i = st + F_B1I * ps;
+    const double* B2i = st + F_B2I * ps;
+    auto b1h = [&](size_t kk) { return B1i[kk] - hdtodx2 * (emf[kk + pitch] - emf[kk]); };
+    auto b2h = [&](size_t kk) { return B2i[kk] + hdtodx1 * (emf[kk + 1] - emf[kk]); };
+    const double* F1 = sc + S_F1 * ps;
+    const double* F2 = sc + S_F2 * ps;
+    if (i >= g.is) {
+      Cons7 L = get7(sc + S_UL1 * ps, ps, k);
+      Cons7 R = get7(sc + S_UR1 * ps, ps, k);
+      {  // Step 5a: x2 fluxes at (j+1,i-1),(j,i-1) for L and (j+1,i),(j,i) for R
+        const size_t a1 = k + pitch - 1, a0 = k - 1, c1 = k + pitch, c0 = k;
+        L.d = L.d - hdtodx2 * (F2[a1] - F2[a0]);
+        L.mx = L.mx - hdtodx2 * (F2[3 * ps + a1] - F2[3 * ps + a0]);
+        L.my = L.my - hdtodx2 * (F2[1 * ps + a1] - F2[1 * ps + a0]);
+        L.mz = L.mz - hdtodx2 * (F2[2 * ps + a1] - F2[2 * ps + a0]);
+        L.e = L.e - hdtodx2 * (F2[4 * ps + a1] - F2[4 * ps + a0]);
+        L.bz = L.bz - hdtodx2 * (F2[5 * ps + a1] - F2[5 * ps + a0]);
+        R.d = R.d - hdtodx2 * (F2[c1] - F2[c0]);
+        R.mx = R.mx - hdtodx2 * (F2[3 * ps + c1] - F2[3 * ps + c0]);
+        R.my = R.my - hdtodx2 * (F2[1 * ps + c1] - F2[1 * ps + c0]);
+        R.mz = R.mz - hdtodx2 * (F2[2 * ps + c1] - F2[2 * ps + c0]);
+        R.e = R.e - hdtodx2 * (F2[4 * ps + c1] - F2[4 * ps + c0]);
+        R.bz = R.bz - hdtodx2 * (F2[5 * ps + c1] - F2[5 * ps + c0]);
+      }
+      {  // Step 5b: MHD source terms with the cell-centred field of cells i-1, i
+        double dbx = B1i[k] - B1i[k - 1];
+        double b1 = st[F_B1C * ps + k - 1], b2 = st[F_B2C * ps + k - 1], b3 = st[F_B3C * ps + k - 1];
+        double v3 = st[F_M3 * ps + k - 1] / st[F_D * ps + k - 1];
+        L.mx = L.mx + hdtodx1 * b1 * dbx;
+        L.my = L.my + hdtodx1 * b2 * dbx;
+        L.mz = L.mz + hdtodx1 * b3 * dbx;
+        L.bz = L.bz + hdtodx1 * v3 * dbx;
+        L.e = L.e + hdtodx1 * b3 * v3 * dbx;
+        dbx = B1i[k + 1] - B1i[k];
+        b1 = st[F_B1C * ps + k]; b2 = st[F_B2C * ps + k]; b3 = st[F_B3C * ps + k];
+        v3 = st[F_M3 * ps + k] / st[F_D * ps + k];
+        R.mx = R.mx + hdtodx1 * b1 * dbx;
+        R.my = R.my + hdtodx1 * b2 * dbx;
+        R.mz = R.mz + hdtodx1 * b3 * dbx;
+        R.bz = R.bz + hdtodx1 * v3 * dbx;
+        R.e = R.e + hdtodx1 * b3 * v3 * dbx;
+      }
+      const double bx = b1h(k);
+      const Prim7 wl = cons_to_prim(L, bx, eos);
+      const Prim7 wr = cons_to_prim(R, bx, eos);
+      put7(sc + S_F1C * ps, ps, k, riemann(g.flux, L, R, wl, wr, bx, eos));
+    }
+    if (j >= g.js) {
+      Cons7 L = get7(sc + S_UL2 * ps, ps, k);
+      Cons7 R = get7(sc + S_UR2 * ps, ps, k);
+      {  // Step 6a: x1 fluxes at (j-1,i+1),(j-1,i) for L and (j,i+1),(j,i) for R
+        const size_t a1 = k - pitch + 1, a0 = k - pitch, c1 = k + 1, c0 = k;
+        L.d = L.d - hdtodx1 * (F1[a1] - F1[a0]);
+        L.mx = L.mx - hdtodx1 * (F1[2 * ps + a1] - F1[2 * ps + a0]);
+        L.my = L.my - hdtodx1 * (F1[3 * ps + a1] - F1[3 * ps + a0]);
+        L.mz = L.mz - hdtodx1 * (F1[1 * ps + a1] - F1[1 * ps + a0]);
+        L.e = L.e - hdtodx1 * (F1[4 * ps + a1] - F1[4 * ps + a0]);
+        L.by = L.by - hdtodx1 * (F1[6 * ps + a1] - F1[6 * ps + a0]);
+        R.d = R.d - hdtodx1 * (F1[c1] - F1[c0]);
+        R.mx = R.mx - hdtodx1 * (F1[2 * ps + c1] - F1[2 * ps + c0]);
+        R.my = R.my - hdtodx1 * (F1[3 * ps + c1] - F1[3 * ps + c0]);
+        R.mz = R.mz - hdtodx1 * (F1[1 * ps + c1] - F1[1 * ps + c0]);
+        R.e = R.e - hdtodx1 * (F1[4 * ps + c1] - F1[4 * ps + c0]);
+        R.by = R.by - hdtodx1 * (F1[6 * ps + c1] - F1[6 * ps + c0]);
+      }
+      {  // Step 6b: MHD source terms with the cell-centred field of cells j-1, j
+        const size_t km = k - pitch;
+        double dby = B2i[k] - B2i[km];
+        double b1 = st[F_B1C * ps + km], b2 = st[F_B2C * ps + km], b3 = st[F_B3C * ps + km];
+        double v3 = st[F_M3 * ps + km] / st[F_D * ps + km];
+        L.mz = L.mz + hdtodx2 * b1 * dby;
+        L.mx = L.mx + hdtodx2 * b2 * dby;
+        L.my = L.my + hdtodx2 * b3 * dby;
+        L.by = L.by + hdtodx2 * v3 * dby;
+        L.e = L.e + hdtodx2 * b3 * v3 * dby;
+        dby = B2i[k + pitch] - B2i[k];
+        b1 = st[F_B1C * ps + k]; b2 = st[F_B2C * ps + k]; b3 = st[F_B3C * ps + k];
+        v3 = st[F_M3 * ps + k] / st[F_D * ps + k];
+        R.mz = R.mz + hdtodx2 * b1 * dby;
+        R.mx = R.mx + hdtodx2 * b2 * dby;
+        R.my = R.my + hdtodx2 * b3 * dby;
+        R.by = R.by + hdtodx2 * v3 * dby;
+        R.e = R.e + hdtodx2 * b3 * v3 * dby;
+      }
+      const double bx = b2h(k);
+      const Prim7 wl = cons_to_prim(L, bx, eos);
+      const Prim7 wr = cons_to_prim(R, bx, eos);
+      put7(sc + S_F2C * ps, ps, k, riemann(g.flux, L, R, wl, wr, bx, eos));
+    }
+    // Step 7
+    const double dhalf = st[F_D * ps + k] - hdtodx1 * (F1[k + 1] - F1[k]) -
+                         hdtodx2 * (F2[k + pitch] - F2[k]);
+    const double m1 = st[F_M1 * ps + k] - hdtodx1 * (F1[1 * ps + k + 1] - F1[1 * ps + k]) -
+                      hdtodx2 * (F2[3 * ps + k + pitch] - F2[3 * ps + k]);
+    const double m2 = st[F_M2 * ps + k] - hdtodx1 * (F1[2 * ps + k + 1] - F1[2 * ps + k]) -
+                      hdtodx2 * (F2[1 * ps + k + pitch] - F2[1 * ps + k]);
+    const double b1c = 0.5 * (b1h(k) + b1h(k + 1));
+    const double b2c = 0.5 * (b2h(k) + b2h(k + pitch));
+    sc[S_ECH * ps + k] = (b1c * m2 - b2c * m1) / dhalf;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// new_dt.c:35-90 for one cell (the sound-speed form the reference keeps)
+__device__ __forceinline__ double inverse_dt(const Inst& g, double d, double M1, double M2,
+                                             double M3, double E) {
+  const double di = 1.0 / d;
+  const double v1 = M1 * di, v2 = M2 * di, v3 = M3 * di;
+  const double qsq = v1 * v1 + v2 * v2 + v3 * v3;
+  const double p = dmax(g.g1 * (E - 0.5 * d * qsq), kTiny);
+  const double asq = g.gamma * p * di;
+  double m = 0.0;
+  if (g.nx1 > 1) m = dmax(m, (fabs(v1) + sqrt(asq)) / g.dx1);
+  if (g.nx2 > 1) m = dmax(m, (fabs(v2) + sqrt(asq)) / g.dx2);
+  return m;
+}
+
+// Steps 9, 11a, 11b, 13 (:473-534) over [js,je+1]x[is,ie+1] into the other
+// ping-pong buffer, and the new_dt maximum of the updated interior. The B2c
+// update of 11a and the B1c update of 11b are dropped: Step 13 overwrites them.
+__global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict__ insts,
+                                                           int ninst, int ntiles) {
+  __shared__ double red[4];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Inst& g = insts[find_inst(insts, ninst, K_FINAL, t)];
+    Clock* clk = g.clk;
+    if (!clk->active) continue;
+    const int local = t - g.tile0[K_FINAL];
+    const int ntx = dcdiv(g.nx1 + 1, kTileX);
+    const int ty = local / ntx, tx = local - ty * ntx;
+    const int i = g.is + tx * kTileX + threadIdx.x;
+    const int j = g.js + ty * kTileY + threadIdx.y;
+    double dti = 0.0;
+    if (i <= g.ie + 1 && j <= g.je + 1) {
+      const double dt = clk->dt;
+      const double dtodx1 = dt / g.dx1, dtodx2 = dt / g.dx2;
+      const int cur = clk->cur;
+      const double* __restrict__ st = g.state[cur];
+      double* __restrict__ out = g.state[cur ^ 1];
+      const double* __restrict__ sc = g.scratch;
+      const size_t ps = g.ps;
+      const int pitch = g.pitch;
+      const size_t k = (size_t)j * pitch + i;
+      const double* emf = sc + S_EMF * ps;
+      const double e00 = emf[k], e10 = emf[k + pitch], e01 = emf[k + 1];
+      double b1n = 0.0, b2n = 0.0;
+      if (j <= g.je) {
+        b1n = st[F_B1I * ps + k] - dtodx2 * (e10 - e00);
+        out[F_B1I * ps + k] = b1n;
+      }
+      if (i <= g.ie) {
+        b2n = st[F_B2I * ps + k] + dtodx1 * (e01 - e00);
+        out[F_B2I * ps + k] = b2n;
+      }
+      if (j <= g.je && i <= g.ie) {
+        const double e11 = emf[k + pitch + 1];
+        const double b1p = st[F_B1I * ps + k + 1] - dtodx2 * (e11 - e01);
+        const double b2p = st[F_B2I * ps + k + pitch] + dtodx1 * (e11 - e10);
+        const double* F1 = sc + S_F1C * ps;
+        const double* F2 = sc + S_F2C * ps;
+        double d = st[F_D * ps + k], M1 = st[F_M1 * ps + k], M2 = st[F_M2 * ps + k];
+        double M3 = st[F_M3 * ps + k], E = st[F_E * ps + k], B3c = st[F_B3C * ps + k];
+        d = d - dtodx1 * (F1[k + 1] - F1[k]);
+        M1 = M1 - dtodx1 * (F1[1 * ps + k + 1] - F1[1 * ps + k]);
+        M2 = M2 - dtodx1 * (F1[2 * ps + k + 1] - F1[2 * ps + k]);
+        M3 = M3 - dtodx1 * (F1[3 * ps + k + 1] - F1[3 * ps + k]);
+        E = E - dtodx1 * (F1[4 * ps + k + 1] - F1[4 * ps + k]);
+        B3c = B3c - dtodx1 * (F1[6 * ps + k + 1] - F1[6 * ps + k]);
+        const size_t kp = k + pitch;
+        d = d - dtodx2 * (F2[kp] - F2[k]);
+        M1 = M1 - dtodx2 * (F2[3 * ps + kp] - F2[3 * ps + k]);
+        M2 = M2 - dtodx2 * (F2[1 * ps + kp] - F2[1 * ps + k]);
+        M3 = M3 - dtodx2 * (F2[2 * ps + kp] - F2[2 * ps + k]);
+        E = E - dtodx2 * (F2[4 * ps + kp] - F2[4 * ps + k]);
+        B3c = B3c - dtodx2 * (F2[5 * ps + kp] - F2[5 * ps + k]);
+        out[F_D * ps + k] = d;
+        out[F_M1 * ps + k] = M1;
+        out[F_M2 * ps + k] = M2;
+        out[F_M3 * ps + k] = M3;
+        out[F_E * ps + k] = E;
+        out[F_B1C * ps + k] = 0.5 * (b1n + b1p);
+        out[F_B2C * ps + k] = 0.5 * (b2n + b2p);
+        out[F_B3C * ps + k] = B3c;
+        out[F_B3I * ps + k] = B3c;
+        dti = inverse_dt(g, d, M1, M2, M3, E);
+      }
+    }
+    const double m = block_max256(dti, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) atomic_max_pos(&clk->dti_slot[t % kDtSlots], m);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// set_bvals (set_bvals.c:114-145): the x1 fills then the x2 fills compose into
+// one map from each ghost cell to the interior cell it copies, so a single
+// pass reads only non-ghost cells. Per plane, the exceptions of the reference
+// hold: B1i[ie+1] and B2i[je+1] are not boundary values (:630, :873, :1011);
+// B2i's x1 fills cover rows js..je+1; reflection negates M1 across x1 and M2
+// across x2 (:312-545), outflow copies the edge cell (:553-786), periodic
+// wraps (:793-1031).
+__device__ __forceinline__ bool ghost_src(const Inst& g, int f, int j, int i, int& sj, int& si,
+                                          bool& neg) {
+  const bool b1 = (f == F_B1I), b2 = (f == F_B2I);
+  const int ie_x = b1 ? g.ie + 1 : g.ie;  // last non-ghost column of this plane
+  const int je_y = b2 ? g.je + 1 : g.je;  // last non-ghost row
+  const bool gx = (i < g.is) || (i > ie_x);
+  const bool gy = (j < g.js) || (j > je_y);
+  bool nx = false, ny = false;
+  auto map_x = [&](int ii) -> int {
+    if (ii < g.is) {
+      const int kk = g.is - ii;
+      if (g.bc[0] == BC_PERIODIC) return g.ie - (kk - 1);
+      if (g.bc[0] == BC_OUTFLOW) return g.is;
+      nx = true;
+      return g.is + (kk - 1);
+    }
+    const int kk = ii - g.ie;
+    if (g.bc[1] == BC_PERIODIC) return g.is + (kk - 1);
+    if (g.bc[1] == BC_OUTFLOW) return g.ie;
+    nx = true;
+    return b1 ? g.ie - (kk - 2) : g.ie - (kk - 1);
+  };
+  auto map_y = [&](int jj) -> int {
+    if (jj < g.js) {
+      const int kk = g.js - jj;
+      if (g.bc[2] == BC_PERIODIC) return g.je - (kk - 1);
+      if (g.bc[2] == BC_OUTFLOW) return g.js;
+      ny = true;
+      return g.js + (kk - 1);
+    }
+    const int kk = jj - g.je;
+    if (g.bc[3] == BC_PERIODIC) return g.js + (kk - 1);
+    if (g.bc[3] == BC_OUTFLOW) return g.je;
+    ny = true;
+    return b2 ? g.je - (kk - 2) : g.je - (kk - 1);
+  };
+  if (gy) {
+    sj = map_y(j);
+    si = gx ? map_x(i) : i;
+  } else if (gx && j >= g.js && j <= je_y) {
+    sj = j;
+    si = map_x(i);
+  } else {
+    return false;
+  }
+  neg = (f == F_M1 && nx) || (f == F_M2 && ny);
+  return true;
+}
+
+// which = 0: fill the current state buffer; 1: fill the other one (the state
+// being produced by this step); 2: carry the current buffer's ghost values
+// unchanged into the other one (an in-place integrate_2d leaves them as they
+// were). respect_active = 0 runs regardless of the clock.
+__global__ void __launch_bounds__(kGhostBlock) k_ghost(const Inst* __restrict__ insts, int ninst,
+                                                       int ntiles, int which, int respect_active) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Inst& g = insts[find_inst(insts, ninst, K_GHOST, t)];
+    const Clock* clk = g.clk;
+    if (respect_active && !clk->active) continue;
+    const int e = (t - g.tile0[K_GHOST]) * kGhostBlock + threadIdx.x;
+    const int nrow = 8 * g.sx;
+    if (e >= nrow + (g.nx2 + 1) * 8) continue;
+    int j, i;
+    if (e < nrow) {
+      const int r = e / g.sx;
+      i = e - r * g.sx;
+      j = (r < 4) ? r : g.je + 1 + (r - 4);
+    } else {
+      const int e2 = e - nrow;
+      const int r = e2 >> 3, c = e2 & 7;
+      j = g.js + r;
+      i = (c < 4) ? c : g.ie + 1 + (c - 4);
+    }
+    double* buf = g.state[which ? (clk->cur ^ 1) : clk->cur];
+    const double* src = (which == 2) ? g.state[clk->cur] : buf;
+    const size_t ps = g.ps;
+    const size_t k = (size_t)j * g.pitch + i;
+#pragma unroll
+    for (int f = 0; f < NSTATE; f++) {
+      int sj, si;
+      bool neg;
+      if (ghost_src(g, f, j, i, sj, si, neg)) {
+        if (which == 2) {
+          buf[f * ps + k] = src[f * ps + k];
+        } else {
+          const double v = src[f * ps + (size_t)sj * g.pitch + si];
+          buf[f * ps + k] = neg ? -v : v;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Clock kernels: one workgroup of 256 threads per instance.
+// mode 0: main-loop step (main.c:221-232): run while time < tlim and
+//         (nlim < 0 or nstep < nlim); clamp dt to reach tlim exactly.
+// mode 1: unconditional single integration (drop-in integrate_2d).
+__global__ void __launch_bounds__(256) k_begin(const Inst* __restrict__ insts, int mode) {
+  Clock* c = insts[blockIdx.x].clk;
+  for (int s = threadIdx.x; s < kDtSlots; s += blockDim.x) c->dti_slot[s] = 0ull;
+  if (threadIdx.x == 0) {
+    int act = 1;
+    if (mode == 0) {
+      act = (c->time < c->tlim) && (c->nlim < 0 || c->nstep < c->nlim);
+      if (act && (c->tlim - c->time) < c->dt) c->dt = (c->tlim - c->time);
+    }
+    c->active = act;
+  }
+}
+
+__device__ __forceinline__ double reduce_slots(const Clock* c, double* red) {
+  double m = __longlong_as_double((long long)c->dti_slot[threadIdx.x]);
+  return block_max256(m, red);
+}
+
+// mode 0 (end of a main-loop step, main.c:289-292 + new_dt.c:93-97):
+//   nstep++, time += dt, dt = MIN(2 dt, CourNo/max_dti), swap buffers
+// mode 1 (end of an unconditional integration): swap buffers only
+__global__ void __launch_bounds__(kDtSlots) k_end(const Inst* __restrict__ insts, int mode) {
+  __shared__ double red[4];
+  Clock* c = insts[blockIdx.x].clk;
+  const double max_dti = reduce_slots(c, red);
+  if (threadIdx.x == 0 && c->active) {
+    if (mode == 0) {
+      c->nstep = c->nstep + 1;
+      c->time = c->time + c->dt;
+      const double a = 2.0 * c->dt, b = c->cour_no / max_dti;
+      c->dt = (c->nstep == 0) ? b : dmin(a, b);
+      c->steps_run = c->steps_run + 1;
+    }
+    c->cur = c->cur ^ 1;
+  }
+}
+
+// standalone new_dt (new_dt.c:25-103) on the current state
+__global__ void __launch_bounds__(kTileX* kTileY) k_dti(const Inst* __restrict__ insts, int ninst,
+                                                         int ntiles) {
+  __shared__ double red[4];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Inst& g = insts[find_inst(insts, ninst, K_INTERIOR, t)];
+    Clock* clk = g.clk;
+    const int local = t - g.tile0[K_INTERIOR];
+    const int ntx = dcdiv(g.nx1, kTileX);
+    const int ty = local / ntx, tx = local - ty * ntx;
+    const int i = g.is + tx * kTileX + threadIdx.x;
+    const int j = g.js + ty * kTileY + threadIdx.y;
+    double dti = 0.0;
+    if (i <= g.ie && j <= g.je) {
+      const double* st = g.state[clk->cur];
+      const size_t ps = g.ps, k = (size_t)j * g.pitch + i;
+      dti = inverse_dt(g, st[F_D * ps + k], st[F_M1 * ps + k], st[F_M2 * ps + k],
+                       st[F_M3 * ps + k], st[F_E * ps + k]);
+    }
+    const double m = block_max256(dti, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) atomic_max_pos(&clk->dti_slot[t % kDtSlots], m);
+  }
+}
+
+__global__ void __launch_bounds__(kDtSlots) k_dt_final(const Inst* __restrict__ insts) {
+  __shared__ double red[4];
+  Clock* c = insts[blockIdx.x].clk;
+  const double max_dti = reduce_slots(c, red);
+  if (threadIdx.x == 0) {
+    const double b = c->cour_no / max_dti;
+    c->dt = (c->nstep == 0) ? b : dmin(2.0 * c->dt, b);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_clear_slots(const Inst* __restrict__ insts) {
+  Clock* c = insts[blockIdx.x].clk;
+  for (int s = threadIdx.x; s < kDtSlots; s += blockDim.x) c->dti_slot[s] = 0ull;
+  if (threadIdx.x == 0) c->divb_bits = 0ull;
+}
+
+// max over the interior of |div B| = |dB1/dx1 + dB2/dx2| on the current state
+__global__ void __launch_bounds__(kTileX* kTileY) k_divb(const Inst* __restrict__ insts, int ninst,
+                                                          int ntiles) {
+  __shared__ double red[4];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Inst& g = insts[find_inst(insts, ninst, K_INTERIOR, t)];
+    Clock* clk = g.clk;
+    const int local = t - g.tile0[K_INTERIOR];
+    const int ntx = dcdiv(g.nx1, kTileX);
+    const int ty = local / ntx, tx = local - ty * ntx;
+    const int i = g.is + tx * kTileX + threadIdx.x;
+    const int j = g.js + ty * kTileY + threadIdx.y;
+    double v = 0.0;
+    if (i <= g.ie && j <= g.je) {
+      const double* st = g.state[clk->cur];
+      const size_t ps = g.ps, k = (size_t)j * g.pitch + i;
+      v = fabs((st[F_B1I * ps + k + 1] - st[F_B1I * ps + k]) / g.dx1 +
+               (st[F_B2I * ps + k + g.pitch] - st[F_B2I * ps + k]) / g.dx2);
+    }
+    const double m = block_max256(v, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) atomic_max_pos(&clk->divb_bits, m);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers of this variant
+static int grid_of(const BatchLaunch& b, int kind, int per_cu) {
+  const int n = b.ntiles[kind];
+  if (b.shape.persist <= 0) return n;
+  const int cap = b.ncu * b.shape.persist * per_cu;
+  return n < cap ? n : cap;
+}
+
+}  // namespace ATHMI_VARIANT
+}  // namespace athmi
+
+#define ATHMI_CAT2(a, b) a##b
+#define ATHMI_CAT(a, b) ATHMI_CAT2(a, b)
+#define ATHMI_FN(name) ATHMI_CAT(ATHMI_CAT(athmi_, ATHMI_VARIANT), _##name)
+
+using namespace athmi;
+using namespace athmi::ATHMI_VARIANT;
+
+extern "C" {
+
+// one integration of every active instance (kernels 2-7 of the pipeline)
+hipError_t ATHMI_FN(launch_integrate)(const void* bl, hipStream_t s) {
+  const BatchLaunch& b = *static_cast<const BatchLaunch*>(bl);
+  const dim3 b2(kTileX, kTileY);
+  hipLaunchKernelGGL(k_pred_x1, dim3(grid_of(b, K_PRED_X1, 4)), dim3(kX1Block), 0, s, b.insts,
+                     b.ninst, b.ntiles[K_PRED_X1]);
+  hipLaunchKernelGGL(k_pred_x2, dim3(grid_of(b, K_PRED_X2, 16)), dim3(kX2Block), 0, s, b.insts,
+                     b.ninst, b.ntiles[K_PRED_X2], b.shape.x2_seg);
+  hipLaunchKernelGGL(k_corner<true>, dim3(grid_of(b, K_CORNER_P, 8)), b2, 0, s, b.insts, b.ninst,
+                     b.ntiles[K_CORNER_P]);
+  hipLaunchKernelGGL(k_corr, dim3(grid_of(b, K_CORR, 4)), b2, 0, s, b.insts, b.ninst,
+                     b.ntiles[K_CORR]);
+  hipLaunchKernelGGL(k_corner<false>, dim3(grid_of(b, K_CORNER_C, 8)), b2, 0, s, b.insts,
+                     b.ninst, b.ntiles[K_CORNER_C]);
+  hipLaunchKernelGGL(k_final, dim3(grid_of(b, K_FINAL, 8)), b2, 0, s, b.insts, b.ninst,
+                     b.ntiles[K_FINAL]);
+  return hipGetLastError();
+}
+
+hipError_t ATHMI_FN(launch_begin)(const void* bl, int mode, hipStream_t s) {
+  const BatchLaunch& b = *static_cast<const BatchLaunch*>(bl);
+  hipLaunchKernelGGL(k_begin, dim3(b.ninst), dim3(256), 0, s, b.insts, mode);
+  return hipGetLastError();
+}
+
+hipError_t ATHMI_FN(launch_end)(const void* bl, int mode, hipStream_t s) {
+  const BatchLaunch& b = *static_cast<const BatchLaunch*>(bl);
+  hipLaunchKernelGGL(k_end, dim3(b.ninst), dim3(kDtSlots), 0, s, b.insts, mode);
+  return hipGetLastError();
+}
+
+hipError_t ATHMI_FN(launch_ghost)(const void* bl, int which, int respect_active, hipStream_t s) {
+  const BatchLaunch& b = *static_cast<const BatchLaunch*>(bl);
+  hipLaunchKernelGGL(k_ghost, dim3(grid_of(b, K_GHOST, 8)), dim3(kGhostBlock), 0, s, b.insts,
+                     b.ninst, b.ntiles[K_GHOST], which, respect_active);
+  return hipGetLastError();
+}
+
+hipError_t ATHMI_FN(launch_new_dt)(const void* bl, hipStream_t s) {
+  const BatchLaunch& b = *static_cast<const BatchLaunch*>(bl);
+  hipLaunchKernelGGL(k_clear_slots, dim3(b.ninst), dim3(256), 0, s, b.insts);
+  hipLaunchKernelGGL(k_dti, dim3(grid_of(b, K_INTERIOR, 8)), dim3(kTileX, kTileY), 0, s, b.insts,
+                     b.ninst, b.ntiles[K_INTERIOR]);
+  hipLaunchKernelGGL(k_dt_final, dim3(b.ninst), dim3(kDtSlots), 0, s, b.insts);
+  return hipGetLastError();
+}
+
+hipError_t ATHMI_FN(launch_divb)(const void* bl, hipStream_t s) {
+  const BatchLaunch& b = *static_cast<const BatchLaunch*>(bl);
+  hipLaunchKernelGGL(k_clear_slots, dim3(b.ninst), dim3(256), 0, s, b.insts);
+  hipLaunchKernelGGL(k_divb, dim3(grid_of(b, K_INTERIOR, 8)), dim3(kTileX, kTileY), 0, s,
+                     b.insts, b.ninst, b.ntiles[K_INTERIOR]);
+  return hipGetLastError();
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// step_kernels.hpp -- data layout shared by the host side and the HIP kernels
+// of the CTU+CT step. One "instance" is one 2-D grid (the reference's Grid,
+// athena.h:106-120); the plain step is a batch of one instance.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+namespace athmi {
+
+// field planes of a state buffer, in Gas order (athena.h:43-52) + interface B
+enum Field { F_D = 0, F_M1, F_M2, F_M3, F_E, F_B1C, F_B2C, F_B3C, F_B1I, F_B2I, F_B3I, NSTATE = 11 };
+
+// scratch planes (each Cons1D-shaped group is 7 planes d,Mx,My,Mz,E,By,Bz)
+enum Scratch {
+  S_UL1 = 0,    // x1-face left states (conserved), predictor
+  S_UR1 = 7,    // x1-face right states
+  S_F1 = 14,    // x1 predictor fluxes
+  S_UL2 = 21,   // x2-face left states (x2 sweep order d,M2,M3,M1,E,B3c,B1c)
+  S_UR2 = 28,
+  S_F2 = 35,    // x2 predictor fluxes
+  S_F1C = 42,   // x1 corrector fluxes
+  S_F2C = 49,   // x2 corrector fluxes
+  S_ECC = 56,   // cell-centred emf3 at t^n
+  S_EMF = 57,   // corner emf3 (predictor, then corrector)
+  S_ECH = 58,   // cell-centred emf3 at t^{n+1/2}
+  NSCRATCH = 59
+};
+
+enum BcKind { BC_REFLECT = 1, BC_OUTFLOW = 2, BC_PERIODIC = 4 };
+
+constexpr int kDtSlots = 256;  // partial max slots of the fused dt reduction
+
+// Device-resident clock of one instance (Grid.dt/time/nstep of athena.h:113-114
+// plus the main-loop limits of main.c:217-218).
+struct Clock {
+  double dt, time, tlim, cour_no;
+  int nstep, nlim, active, cur;  // cur: which ping-pong state buffer holds the state
+  int steps_run, pad0, pad1, pad2;
+  unsigned long long divb_bits;           // max |div B| (bit pattern), diagnostics
+  unsigned long long dti_slot[kDtSlots];  // bit patterns of non-negative doubles
+};
+
+struct Inst {
+  int nx1, nx2, is, ie, js, je, sx, sy, pitch, flux;
+  int bc[4];          // ix1, ox1, ix2, ox2
+  double dx1, dx2, gamma, g1, g2;
+  double* state[2];   // two ping-pong buffers of NSTATE planes
+  double* scratch;    // NSCRATCH planes
+  size_t ps;          // plane stride in doubles
+  Clock* clk;
+  // first global tile of this instance for each kernel kind
+  int tile0[8];
+};
+
+// Tile kinds (one per kernel family); tile counts per instance come from
+// tiles_of() below so host and device agree on the decomposition.
+enum KernelKind {
+  K_PRED_X1 = 0,  // rows js-2..je+2, faces is-1..ie+2 in segments of kX1Faces
+  K_PRED_X2,      // columns is-2..ie+2 in chunks of kX2Block, faces js-1..je+2 in segments
+  K_CORR,         // cells [js-1,je+1] x [is-1,ie+1]
+  K_CORNER_P,     // corners [js-1,je+2] x [is-1,ie+2]
+  K_CORNER_C,     // corners [js,je+1] x [is,ie+1]
+  K_FINAL,        // faces/cells [js,je+1] x [is,ie+1]
+  K_GHOST,        // ghost cells of all planes
+  K_INTERIOR,     // cells [js,je] x [is,ie]
+  NKINDS
+};
+
+// Work decomposition (tile sizes) shared by host and device
+constexpr int kX1Block = 256;               // threads of a predictor-x1 tile
+constexpr int kX1Faces = kX1Block - 3;      // faces produced per x1 tile
+constexpr int kX2Block = 64;                // columns of a predictor-x2 tile
+constexpr int kTileX = 64, kTileY = 4;      // 2-D tiles of the cell/corner kernels
+constexpr int kGhostBlock = 256;
+
+inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// number of tiles of kind k for a grid of nx1 x nx2 zones (x2seg: faces
+// marched per thread by the x2 predictor)
+inline int tiles_of(int k, int nx1, int nx2, int x2seg) {
+  const int sx = nx1 + 8;
+  switch (k) {
+    case K_PRED_X1: return (nx2 + 4) * cdiv(nx1 + 4, kX1Faces);
+    case K_PRED_X2: return cdiv(nx1 + 5, kX2Block) * cdiv(nx2 + 4, x2seg);
+    case K_CORR: return cdiv(nx1 + 3, kTileX) * cdiv(nx2 + 3, kTileY);
+    case K_CORNER_P: return cdiv(nx1 + 4, kTileX) * cdiv(nx2 + 4, kTileY);
+    case K_CORNER_C: return cdiv(nx1 + 2, kTileX) * cdiv(nx2 + 2, kTileY);
+    case K_FINAL: return cdiv(nx1 + 1, kTileX) * cdiv(nx2 + 1, kTileY);
+    case K_GHOST: return cdiv(8 * sx + (nx2 + 1) * 8, kGhostBlock);
+    default: return cdiv(nx1, kTileX) * cdiv(nx2, kTileY);
+  }
+}
+
+// Launch shape chosen by the autotuner
+struct LaunchShape {
+  int x2_seg;        // faces marched per thread in the x2 predictor
+  int persist;       // 0: one tile per workgroup; >0: workgroups per CU of a persistent grid
+};
+
+// Everything a launcher needs for one batch (device descriptor array, tile
+// totals per kernel kind, the tuned shape and the CU count)
+struct BatchLaunch {
+  const Inst* insts;
+  int ninst;
+  int ntiles[NKINDS];
+  LaunchShape shape;
+  int ncu;
+};
+
+}  // namespace athmi
+
+// Launchers exported by each arithmetic variant of step_kernels.hip
+// (athmi_exact_* built with -ffp-contract=off, athmi_fast_* with FMA).
+#define ATHMI_DECLARE_VARIANT(V)                                                              \
+  extern "C" hipError_t athmi_##V##_launch_integrate(const void* bl, hipStream_t s);          \
+  extern "C" hipError_t athmi_##V##_launch_begin(const void* bl, int mode, hipStream_t s);    \
+  extern "C" hipError_t athmi_##V##_launch_end(const void* bl, int mode, hipStream_t s);      \
+  extern "C" hipError_t athmi_##V##_launch_ghost(const void* bl, int which, int respect,      \
+                                                 hipStream_t s);                              \
+  extern "C" hipError_t athmi_##V##_launch_new_dt(const void* bl, hipStream_t s);             \
+  extern "C" hipError_t athmi_##V##_launch_divb(const void* bl, hipStream_t s);

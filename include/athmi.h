@@ -1,0 +1,134 @@
+/*
+ * athmi.h -- C-ABI of the MI355X-native 2-D CTU + constrained-transport MHD
+ * step (the Athena 3.1 path of awasiljew/athena-cuda).
+ *
+ * One handle holds one or more independent grid instances resident in HBM
+ * (a plain grid is a batch of one). Every function returns ATHMI_OK (0) or a
+ * negative error code; athmi_last_error() describes the last failure of the
+ * calling thread. No HIP, CUDA or torch types appear: streams are passed as
+ * void*. Each entry point cites the reference interface it replaces.
+ *
+ * Host arrays use the reference layouts: the conserved state is the Gas
+ * array of athena.h:43-52 (8 doubles per cell: d, M1, M2, M3, E, B1c, B2c,
+ * B3c), row-major [Nx2+8][Nx1+8] including the 4 ghost layers; the
+ * interface fields B1i, B2i, B3i are [Nx2+8][Nx1+8] doubles
+ * (athena.h:106-109, init_grid.c:170-210). Index (j,i) = cell (i,j).
+ */
+#ifndef ATHMI_H
+#define ATHMI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATHMI_OK 0
+#define ATHMI_EINVAL (-1)  /* bad argument */
+#define ATHMI_EHIP (-2)    /* HIP runtime error */
+#define ATHMI_ENOMEM (-3)  /* device or host allocation failed */
+#define ATHMI_ESTATE (-4)  /* call not valid in the handle's current state */
+
+/* boundary flags of the athinput <grid> block (set_bvals.c:26-29) */
+#define ATHMI_BC_REFLECT 1
+#define ATHMI_BC_OUTFLOW 2
+#define ATHMI_BC_PERIODIC 4
+
+#define ATHMI_FLUX_ROE 0  /* flux_roe.c:54 with its HLLE fallback */
+#define ATHMI_FLUX_HLLE 1 /* flux_hlle.c:60 */
+
+#define ATHMI_EXACT 0 /* one rounding per operation (no FMA contraction) */
+#define ATHMI_FAST 1  /* FMA-contracted arithmetic */
+
+typedef struct athmi_grid athmi_grid;
+
+/* Geometry, EOS and options of one instance; mirrors init_grid.c:25-118 plus
+ * the <time>/<problem> constants read by main.c:64-94. */
+typedef struct athmi_desc {
+  int nx1, nx2;                 /* interior zones (Nx1, Nx2 >= 2) */
+  double x1min, x1max, x2min, x2max;
+  double gamma;                 /* Gamma; Gamma_1, Gamma_2 are derived */
+  double cour_no;               /* CourNo */
+  int bc_ix1, bc_ox1, bc_ix2, bc_ox2; /* ATHMI_BC_* */
+  int flux;                     /* ATHMI_FLUX_* */
+  int variant;                  /* ATHMI_EXACT or ATHMI_FAST (same for a whole batch) */
+} athmi_desc;
+
+/* Allocate n instances (device state, scratch and clocks).
+ * Replaces init_grid_gpu + integrate_init_2d_cu + lr_states_init_cu + dt_init
+ * + set_bvals_init_cu (utils_cu.c:132, integrate_2d_cuda.cu:218,333,
+ * new_dt_cuda.cu:241, set_bvals_cuda.cu:591). */
+int athmi_create(const athmi_desc *desc, int n, athmi_grid **out);
+/* Replaces integrate_destruct_2d_cu + dt_destruct + the cudaFree calls of
+ * main.c:321-345. */
+int athmi_destroy(athmi_grid *g);
+int athmi_count(const athmi_grid *g, int *n);
+/* Use exactly these cell sizes (a caller holding a Grid built elsewhere,
+ * whose dx may not round-trip through (max - min)/N) */
+int athmi_set_spacing(athmi_grid *g, int inst, double dx1, double dx2);
+
+/* Host <-> device copies of one instance's full padded state (AoS <-> SoA).
+ * Replace copy_to_gpu_mem / copy_to_host_mem (utils_cu.c:230, 293) without
+ * their 4-cell x shift. */
+int athmi_upload(athmi_grid *g, int inst, const double *gas, const double *b1i,
+                 const double *b2i, const double *b3i);
+int athmi_download(athmi_grid *g, int inst, double *gas, double *b1i, double *b2i,
+                   double *b3i);
+
+/* Grid.dt / Grid.time / Grid.nstep (athena.h:113-114) of one instance */
+int athmi_set_clock(athmi_grid *g, int inst, double dt, double time, int nstep);
+int athmi_get_clock(athmi_grid *g, int inst, double *dt, double *time, int *nstep);
+/* main-loop limits (main.c:217-218: tlim, nlim; nlim < 0 = none); inst = -1: all */
+int athmi_set_limits(athmi_grid *g, int inst, double tlim, int nlim);
+
+/* set_bvals(Grid*, 0) (set_bvals.c:114) on every instance */
+int athmi_set_bvals(athmi_grid *g);
+/* new_dt(Grid*) (new_dt.c:25) on every instance; result in the clock's dt */
+int athmi_new_dt(athmi_grid *g);
+/* integrate_2d(Grid*) (__integrate_2d__.c:100) on every instance with its
+ * clock's dt; the clock's time and nstep are left unchanged */
+int athmi_integrate(athmi_grid *g);
+/* One iteration of the main loop (main.c:221-318) on every instance that is
+ * still running: clamp dt to tlim, integrate, nstep++, time += dt, new_dt,
+ * set_bvals. Everything stays on the device; replayed from a HIP graph. */
+int athmi_step(athmi_grid *g);
+/* Repeat athmi_step until no instance is running or max_steps iterations
+ * (max_steps < 0: no cap); *steps = iterations issued. Synchronises. */
+int athmi_run(athmi_grid *g, int max_steps, int *steps);
+/* max |div B| over the interior of one instance */
+int athmi_divb_max(athmi_grid *g, int inst, double *out);
+
+/* Execution control */
+int athmi_set_stream(athmi_grid *g, void *hip_stream); /* NULL: default stream */
+int athmi_sync(athmi_grid *g);
+/* Launch shape: faces marched per x2-predictor thread, persistent
+ * workgroups per CU (0 = one tile per workgroup). Picked by the autotuner. */
+int athmi_set_shape(athmi_grid *g, int x2_seg, int persist);
+int athmi_use_graph(athmi_grid *g, int on);
+
+/* Device pointers of the current state of one instance: 11 planes
+ * (d, M1, M2, M3, E, B1c, B2c, B3c, B1i, B2i, B3i), each [Nx2+8][pitch] */
+int athmi_device_state(athmi_grid *g, int inst, double **planes, int *pitch);
+
+/* Problem generators (host libm, as the reference), applied to one instance's
+ * interior; follow with athmi_set_bvals and athmi_new_dt as main.c:121-151.
+ * blast.c:25-72 (cx, cy: centre of the over-pressured region, 0,0 in the
+ * reference); field_loop.c:40-137; cpaw2d.c:56-185 (returns the wavelength;
+ * keeps the initial state for athmi_cpaw2d_l1). */
+int athmi_pgen_blast(athmi_grid *g, int inst, double radius, double pamb, double prat,
+                     double b0, double angle_deg, double cx, double cy);
+int athmi_pgen_field_loop(athmi_grid *g, int inst, double rad, double amp, double vflow,
+                          int iprob);
+int athmi_pgen_cpaw2d(athmi_grid *g, int inst, double angle, double b_par, double b_perp,
+                      double pres, double v_par, double *lambda);
+/* cpaw2d.c:222-302: out[0] = RMS error, out[1..8] = L1 error per variable in
+ * Gas order (d, M1, M2, M3, E, B1c, B2c, B3c), each divided by Nx1*Nx2 */
+int athmi_cpaw2d_l1(athmi_grid *g, int inst, double *out);
+
+const char *athmi_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATHMI_H */

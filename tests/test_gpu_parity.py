@@ -1,0 +1,143 @@
+"""GPU parity of the HIP step against the oracle (the CPU restatement that is
+itself bit-exact against the reference's CPU sources, see test_oracle_ref.py).
+
+The exact variant (-ffp-contract=off kernels) must agree with the oracle value
+for value (np.array_equal; -0.0 == 0.0) on every field, ghosts included, and on
+dt/time/nstep. The FMA variant must agree to L1-relative <= 1e-12 after 100
+steps on every conserved and face-B field, the tolerance of the north-star
+parity bar.
+"""
+import numpy as np
+import pytest
+
+from helpers import interior, l1_rel, load_pkg, oracle_setup
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("gas", "B1i", "B2i", "B3i")
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def compare_exact(gpu, ora, tag):
+    a = gpu.download(0)
+    b = ora.get()
+    for name, x, y in zip(FIELDS, a, b):
+        if not np.array_equal(x, y):
+            diff = np.abs(x - y)
+            idx = np.unravel_index(np.argmax(diff), diff.shape)
+            raise AssertionError(f"{tag}: {name} differs, max |diff| {diff.max():.3e} at {idx}")
+    dt_g, t_g, n_g = gpu.clock(0)
+    dt_o, t_o, n_o = ora.clock()
+    assert (dt_g, t_g, n_g) == (dt_o, t_o, n_o), f"{tag}: clock {dt_g, t_g, n_g} vs {dt_o, t_o, n_o}"
+
+
+CASES = [
+    ("field_loop", [], 40),
+    ("blast", ["grid/Nx1=64", "grid/Nx2=96"], 40),
+    ("cpaw2d", ["grid/Nx1=32", "grid/Nx2=32"], 30),
+]
+
+
+@pytest.mark.parametrize("flux", ["roe", "hlle"])
+@pytest.mark.parametrize("problem,overrides,nsteps", CASES)
+def test_exact_steps_match_oracle(problem, overrides, nsteps, flux):
+    pkg = load_pkg()
+    say(f"[gpu] exact {problem} {flux} {overrides} steps={nsteps}")
+    gpu, par, tlim, nlim = pkg.setup_problem(problem, overrides, flux=flux, variant="exact")
+    ora, tlim_o, _ = oracle_setup(problem, overrides, flux=flux)
+    compare_exact(gpu, ora, "initial state")
+    gpu.set_limits(tlim, nsteps)
+    done = 0
+    for chunk in (1, 4, nsteps - 5):
+        for _ in range(chunk):
+            gpu.step()
+            ora.step(tlim)
+        done += chunk
+        compare_exact(gpu, ora, f"after {done} steps")
+    gpu.close()
+
+
+@pytest.mark.parametrize("bcs", [(2, 2, 2, 2), (1, 1, 1, 1), (1, 2, 4, 4), (4, 4, 2, 1)])
+def test_exact_boundaries_match_oracle(bcs):
+    pkg = load_pkg()
+    ov = ["grid/Nx1=48", "grid/Nx2=40", f"grid/ibc_x1={bcs[0]}", f"grid/obc_x1={bcs[1]}",
+          f"grid/ibc_x2={bcs[2]}", f"grid/obc_x2={bcs[3]}", "problem/radius=0.25"]
+    say(f"[gpu] exact blast boundaries {bcs}")
+    gpu, par, tlim, _ = pkg.setup_problem("blast", ov, variant="exact")
+    ora, _, _ = oracle_setup("blast", ov)
+    compare_exact(gpu, ora, "initial ghosts")
+    gpu.set_limits(tlim, 25)
+    for s in range(25):
+        gpu.step()
+        ora.step(tlim)
+    compare_exact(gpu, ora, f"boundaries {bcs} after 25 steps")
+    gpu.close()
+
+
+def test_exact_routines_one_by_one():
+    """integrate_2d, new_dt and set_bvals as separate calls (the drop-in
+    granularity of the reference's main loop)."""
+    pkg = load_pkg()
+    ov = ["grid/Nx1=40", "grid/Nx2=56"]
+    say("[gpu] exact routine-by-routine blast 40x56")
+    gpu, par, tlim, _ = pkg.setup_problem("blast", ov, variant="exact")
+    ora, _, _ = oracle_setup("blast", ov)
+    for s in range(5):
+        gpu.integrate()
+        ora.integrate()
+        compare_exact(gpu, ora, f"integrate #{s}")
+        dt, t, n = gpu.clock(0)
+        gpu.set_clock(0, dt, t + dt, n + 1)
+        ora.set_time(dt, t + dt, n + 1)
+        gpu.new_dt()
+        ora.new_dt()
+        compare_exact(gpu, ora, f"new_dt #{s}")
+        gpu.set_bvals()
+        ora.set_bvals()
+        compare_exact(gpu, ora, f"set_bvals #{s}")
+    gpu.close()
+
+
+@pytest.mark.parametrize("problem,overrides,nsteps", [
+    ("field_loop", [], 100),
+    ("blast", ["grid/Nx1=400", "grid/Nx2=600"], 100),
+])
+@pytest.mark.parametrize("flux", ["roe", "hlle"])
+def test_fast_variant_l1_within_1e12(problem, overrides, nsteps, flux):
+    pkg = load_pkg()
+    say(f"[gpu] fast {problem} {flux} {overrides} steps={nsteps}")
+    gpu, par, tlim, _ = pkg.setup_problem(problem, overrides, flux=flux, variant="fast")
+    ora, _, _ = oracle_setup(problem, overrides, flux=flux)
+    gpu.set_limits(tlim, nsteps)
+    n = gpu.run()
+    assert n >= nsteps
+    ora.run(tlim, nsteps)
+    a = gpu.download(0)
+    b = ora.get()
+    nx1, nx2 = gpu.descs[0]["nx1"], gpu.descs[0]["nx2"]
+    worst = 0.0
+    for q in range(8):
+        worst = max(worst, l1_rel(interior(a[0][..., q], nx1, nx2), interior(b[0][..., q], nx1, nx2)))
+    for k in (1, 2, 3):
+        worst = max(worst, l1_rel(interior(a[k], nx1, nx2), interior(b[k], nx1, nx2)))
+    say(f"    worst L1-relative difference over fields: {worst:.3e}")
+    assert worst <= 1e-12
+    gpu.close()
+
+
+@pytest.mark.parametrize("variant", ["exact", "fast"])
+def test_divb_stays_zero_every_step(variant):
+    pkg = load_pkg()
+    say(f"[gpu] div B {variant} field_loop + blast")
+    for problem, ov in (("field_loop", []), ("blast", ["grid/Nx1=128", "grid/Nx2=192"])):
+        gpu, par, tlim, _ = pkg.setup_problem(problem, ov, variant=variant)
+        gpu.set_limits(tlim, 60)
+        assert gpu.divb_max(0) <= 1e-13
+        for s in range(60):
+            gpu.step()
+            dv = gpu.divb_max(0)
+            assert dv <= 1e-13, f"{problem} step {s}: max|div B| = {dv}"
+        gpu.close()
