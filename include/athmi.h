@@ -105,6 +105,12 @@ int athmi_sync(athmi_grid *g);
  * workgroups per CU (0 = one tile per workgroup). Picked by the autotuner. */
 int athmi_set_shape(athmi_grid *g, int x2_seg, int persist);
 int athmi_use_graph(athmi_grid *g, int on);
+/* Time nsteps main-loop steps kernel by kernel (HIP events on the handle's
+ * stream between consecutive launches, no graph); avg_ms[k] receives the mean
+ * duration of kernel k of the step (nslots >= 9), named by
+ * athmi_step_kernel_name(k). No reference counterpart: measurement hook. */
+int athmi_profile_steps(athmi_grid *g, int nsteps, double *avg_ms, int nslots);
+const char *athmi_step_kernel_name(int k);
 
 /* Device pointers of the current state of one instance: 11 planes
  * (d, M1, M2, M3, E, B1c, B2c, B3c, B1i, B2i, B3i), each [Nx2+8][pitch] */
