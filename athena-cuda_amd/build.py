@@ -43,10 +43,11 @@ def build(verbose_resources=False):
                 cmd.append("-Rpass-analysis=kernel-resource-usage")
             run(cmd)
             objs.append(o)
-        o = os.path.join(tmp, "athmi_capi.o")
-        run([HIPCC, ARCH, *COMMON, *NO_PK, "-ffp-contract=off", "-c",
-             os.path.join(CSRC, "athmi_capi.cpp"), "-o", o])
-        objs.append(o)
+        for src in ("athmi_capi.cpp", "calib.hip"):
+            o = os.path.join(tmp, src.split(".")[0] + ".o")
+            run([HIPCC, ARCH, *COMMON, *NO_PK, "-ffp-contract=off", "-c",
+                 os.path.join(CSRC, src), "-o", o])
+            objs.append(o)
         for src in ("host_problems.cpp", "dropin.cpp"):
             o = os.path.join(tmp, src.replace(".cpp", ".o"))
             run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I", INC, "-I", CSRC,

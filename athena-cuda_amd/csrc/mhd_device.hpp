@@ -734,10 +734,158 @@ ATHMI_HD Cons7 flux_hlle(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl, cons
   return hlle_from_roe(Ul, Ur, Wl, Wr, Bxi, r, S.ev0, S.ev6, eos);
 }
 
+// Scalars of the Roe-matrix eigenvectors that are shared by several rows
+// or columns (esystem_roe.c:129-132 and :218-235)
+struct RoeVecScalars {
+  double afpbb, aspbb;                             // right-eigenvector terms
+  double cff, css, qfn, qsn, afn, asn, afpb, aspb; // left, normalised by 1/2a^2
+  double alpha_fn, alpha_sn, q2_star, q3_star, vqstr, norm2;
+};
+
+ATHMI_HD void roe_vec_scalars(const RoeSpeeds& S, double d, double v2, double v3,
+                              const EosC& eos, RoeVecScalars& V) {
+  V.afpbb = S.af_prime * S.bt_star * S.bet_starsq;
+  V.aspbb = S.as_prime * S.bt_star * S.bet_starsq;
+  double norm = 0.5 / S.twid_asq;
+  V.cff = norm * S.alpha_f * S.cf;
+  V.css = norm * S.alpha_s * S.cs;
+  V.qfn = S.qf * norm;
+  V.qsn = S.qs * norm;
+  V.afn = norm * S.af_prime * d;
+  V.asn = norm * S.as_prime * d;
+  V.afpb = norm * S.af_prime * S.bt_star;
+  V.aspb = norm * S.as_prime * S.bt_star;
+  norm = norm * eos.g1;
+  V.alpha_fn = S.alpha_f * norm;
+  V.alpha_sn = S.alpha_s * norm;
+  V.q2_star = S.bet2_star / S.bet_starsq;
+  V.q3_star = S.bet3_star / S.bet_starsq;
+  V.vqstr = (v2 * V.q2_star + v3 * V.q3_star);
+  V.norm2 = norm * 2.0;
+}
+
+// a = L dU with each left-eigenvector row formed where it is used
+// (esystem_roe.c:237-291; flux_roe.c:200-204 sums m = 0..6 in order)
+ATHMI_HD void roe_project(const RoeSpeeds& S, const RoeVecScalars& V, double v1, double v2,
+                          double v3, double b2, double b3, double x, const EosC& eos,
+                          const double* du, double* a) {
+  const double vsq = S.vsq, hp = S.hp, cf = S.cf, cs = S.cs;
+  const double af = V.alpha_fn, as_ = V.alpha_sn;
+  // rows 0 and 6 share L05 = L65, L06 = L66
+  const double L05 = V.asn * V.q2_star - af * b2;
+  const double L06 = V.asn * V.q3_star - af * b3;
+  const double p5 = L05 * du[5], p6 = L06 * du[6];
+  a[0] = (af * (vsq - hp) + V.cff * (cf + v1) - V.qsn * V.vqstr - V.aspb) * du[0] +
+         (-af * v1 - V.cff) * du[1] + (-af * v2 + V.qsn * V.q2_star) * du[2] +
+         (-af * v3 + V.qsn * V.q3_star) * du[3] + af * du[4] + p5 + p6;
+  a[6] = (af * (vsq - hp) + V.cff * (cf - v1) + V.qsn * V.vqstr - V.aspb) * du[0] +
+         (-af * v1 + V.cff) * du[1] + (-af * v2 - V.qsn * V.q2_star) * du[2] +
+         (-af * v3 - V.qsn * V.q3_star) * du[3] + af * du[4] + p5 + p6;
+  // rows 1 and 5
+  const double q0 = (0.5 * (v2 * S.bet3 - v3 * S.bet2)) * du[0];
+  const double q2 = (-0.5 * S.bet3) * du[2], q3 = (0.5 * S.bet2) * du[3];
+  const double q5 = (-0.5 * S.sqrtd * S.bet3 * S.s) * du[5];
+  const double q6 = (0.5 * S.sqrtd * S.bet2 * S.s) * du[6];
+  a[1] = q0 + q2 + q3 + q5 + q6;
+  a[5] = -q0 - q2 - q3 + q5 + q6;
+  // rows 2 and 4 share L25 = L45, L26 = L46
+  const double s5 = (-V.afn * V.q2_star - as_ * b2) * du[5];
+  const double s6 = (-V.afn * V.q3_star - as_ * b3) * du[6];
+  a[2] = (as_ * (vsq - hp) + V.css * (cs + v1) + V.qfn * V.vqstr + V.afpb) * du[0] +
+         (-as_ * v1 - V.css) * du[1] + (-as_ * v2 - V.qfn * V.q2_star) * du[2] +
+         (-as_ * v3 - V.qfn * V.q3_star) * du[3] + as_ * du[4] + s5 + s6;
+  a[4] = (as_ * (vsq - hp) + V.css * (cs - v1) - V.qfn * V.vqstr + V.afpb) * du[0] +
+         (-as_ * v1 + V.css) * du[1] + (-as_ * v2 + V.qfn * V.q2_star) * du[2] +
+         (-as_ * v3 + V.qfn * V.q3_star) * du[3] + as_ * du[4] + s5 + s6;
+  // row 3
+  const double n2 = V.norm2;
+  a[3] = (1.0 - n2 * (0.5 * vsq - eos.g2 * x / eos.g1)) * du[0] + (n2 * v1) * du[1] +
+         (n2 * v2) * du[2] + (n2 * v3) * du[3] + (-n2) * du[4] + (n2 * b2) * du[5] +
+         (n2 * b3) * du[6];
+}
+
+// column N of the Roe-matrix R (esystem_roe.c:151-213); r[m] = R[m][N]
+template <int N>
+ATHMI_HD void roe_col(const RoeSpeeds& S, const RoeVecScalars& V, double v1, double v2,
+                      double v3, double x, const EosC& eos, double* r) {
+  const double af = S.alpha_f, as_ = S.alpha_s, hp = S.hp;
+  if constexpr (N == 0) {
+    r[0] = af; r[1] = af * S.ev0;
+    r[2] = af * v2 + S.qs * S.bet2_star; r[3] = af * v3 + S.qs * S.bet3_star;
+    r[4] = af * (hp - v1 * S.cf) + S.qs * S.vbet + V.aspbb;
+    r[5] = S.as_prime * S.bet2_star; r[6] = S.as_prime * S.bet3_star;
+  } else if constexpr (N == 1) {
+    r[0] = 0.0; r[1] = 0.0; r[2] = -S.bet3; r[3] = S.bet2;
+    r[4] = -(v2 * S.bet3 - v3 * S.bet2);
+    r[5] = -S.bet3 * S.s * S.isqrtd; r[6] = S.bet2 * S.s * S.isqrtd;
+  } else if constexpr (N == 2) {
+    r[0] = as_; r[1] = as_ * S.ev2;
+    r[2] = as_ * v2 - S.qf * S.bet2_star; r[3] = as_ * v3 - S.qf * S.bet3_star;
+    r[4] = as_ * (hp - v1 * S.cs) - S.qf * S.vbet - V.afpbb;
+    r[5] = -S.af_prime * S.bet2_star; r[6] = -S.af_prime * S.bet3_star;
+  } else if constexpr (N == 3) {
+    r[0] = 1.0; r[1] = v1; r[2] = v2; r[3] = v3;
+    r[4] = 0.5 * S.vsq + eos.g2 * x / eos.g1; r[5] = 0.0; r[6] = 0.0;
+  } else if constexpr (N == 4) {
+    r[0] = as_; r[1] = as_ * S.ev4;
+    r[2] = as_ * v2 + S.qf * S.bet2_star; r[3] = as_ * v3 + S.qf * S.bet3_star;
+    r[4] = as_ * (hp + v1 * S.cs) + S.qf * S.vbet - V.afpbb;
+    r[5] = -S.af_prime * S.bet2_star; r[6] = -S.af_prime * S.bet3_star;
+  } else if constexpr (N == 5) {
+    r[0] = 0.0; r[1] = 0.0; r[2] = S.bet3; r[3] = -S.bet2;
+    r[4] = (v2 * S.bet3 - v3 * S.bet2);
+    r[5] = -S.bet3 * S.s * S.isqrtd; r[6] = S.bet2 * S.s * S.isqrtd;
+  } else {
+    r[0] = af; r[1] = af * S.ev6;
+    r[2] = af * v2 - S.qs * S.bet2_star; r[3] = af * v3 - S.qs * S.bet3_star;
+    r[4] = af * (hp + v1 * S.cf) - S.qs * S.vbet + V.aspbb;
+    r[5] = S.as_prime * S.bet2_star; r[6] = S.as_prime * S.bet3_star;
+  }
+}
+
+// structural non-zeros of the columns (R01, R05, R11, R15, R53, R63 are zero)
+template <int N>
+ATHMI_HD constexpr bool roe_nz(int m) {
+  return !((N == 1 || N == 5) && (m == 0 || m == 1)) && !(N == 3 && (m == 5 || m == 6));
+}
+
+// One column of the Roe-flux sum: the intermediate state u += a[N]*R[*][N] and
+// its positivity test (flux_roe.c:216-231), and f -= coeff[N]*R[*][N]
+// (flux_roe.c:251-256; per component the sum still runs over N ascending).
+// The test is evaluated without branches: the reference stops at the first
+// failing column and takes the HLLE flux, so only whether any column fails
+// matters (a state after a failure is never used).
+template <int N>
+ATHMI_HD void roe_column_pass(const RoeSpeeds& S, const RoeVecScalars& V, double v1,
+                              double v2, double v3, double x, const EosC& eos, const double* a,
+                              const double* coeff, const double* ev, double Bxi, double* ui,
+                              double* f, bool& bad) {
+  double r[7];
+  roe_col<N>(S, V, v1, v2, v3, x, eos, r);
+  if constexpr (N < 6) {
+#pragma unroll
+    for (int m = 0; m < 7; m++)
+      if (roe_nz<N>(m)) ui[m] = ui[m] + (N == 3 && m == 0 ? a[N] : a[N] * r[m]);
+    double p_inter = ui[4] - 0.5 * (sqr(ui[1]) + sqr(ui[2]) + sqr(ui[3])) / ui[0];
+    p_inter = p_inter - 0.5 * (sqr(ui[5]) + sqr(ui[6]) + sqr(Bxi));
+    const bool fails = (ui[0] <= 0.0) || (p_inter < 0.0);
+    bad = bad || ((ev[N + 1] > ev[N]) && fails);
+  }
+#pragma unroll
+  for (int m = 0; m < 7; m++)
+    if (roe_nz<N>(m)) f[m] = f[m] - (N == 3 && m == 0 ? coeff[N] : coeff[N] * r[m]);
+}
+
 // flux_roe.c:54-259, with the intermediate-state test and HLLE fallback
-// (etah = 0, flux_roe.c:40, so the coefficient is |ev|/2 * a)
-ATHMI_HD Cons7 flux_roe(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl, const Prim7& Wr,
-                        double Bxi, const EosC& eos) {
+// (etah = 0, flux_roe.c:40, so the coefficient is |ev|/2 * a).
+//
+// The fallback (flux_roe.c:233-237) is rare; instead of keeping the four
+// input states live through the whole Roe evaluation, it asks `reload` to
+// rebuild them (the callers recompute them from memory with the same
+// arithmetic, so the values are identical) and evaluates flux_hlle afresh.
+template <class Reload>
+ATHMI_HD Cons7 flux_roe_reload(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl,
+                               const Prim7& Wr, double Bxi, const EosC& eos, Reload&& reload) {
   const RoeAvg r = roe_average(Ul, Ur, Wl, Wr, Bxi);
   RoeSpeeds S;
   roe_speeds(r.droe, r.v1, r.v2, r.v3, r.h, Bxi, r.b2, r.b3, r.x, r.y, eos, true, S);
@@ -745,69 +893,61 @@ ATHMI_HD Cons7 flux_roe(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl, const
   const Cons7 Fr = side_flux(Ur, Wr, Bxi, r.pbr);
   if (S.ev0 >= 0.0) return Fl;
   if (S.ev6 <= 0.0) return Fr;
-  double a[7];
+  double f[7];
   {
-    RoeL L;
-    roe_left(S, r.droe, r.v1, r.v2, r.v3, r.b2, r.b3, r.x, eos, L);
-    const double* ul = &Ul.d;
-    const double* ur = &Ur.d;
-    double du[7];
+    const double* fl = &Fl.d;
+    const double* fr = &Fr.d;
 #pragma unroll
-    for (int n = 0; n < 7; n++) du[n] = ur[n] - ul[n];
-    a[0] = L.L00 * du[0] + L.L01 * du[1] + L.L02 * du[2] + L.L03 * du[3] + L.L04 * du[4] +
-           L.L05 * du[5] + L.L06 * du[6];
-    const double q0 = L.L10 * du[0], q2 = L.L12 * du[2], q3 = L.L13 * du[3], q5 = L.L15 * du[5],
-                 q6 = L.L16 * du[6];
-    a[1] = q0 + q2 + q3 + q5 + q6;
-    a[5] = -q0 - q2 - q3 + q5 + q6;
-    const double s5 = L.L25 * du[5], s6 = L.L26 * du[6];
-    a[2] = L.L20 * du[0] + L.L21 * du[1] + L.L22 * du[2] + L.L23 * du[3] + L.L24 * du[4] + s5 + s6;
-    a[3] = L.L30 * du[0] + L.L31 * du[1] + L.L32 * du[2] + L.L33 * du[3] + L.L34 * du[4] +
-           L.L35 * du[5] + L.L36 * du[6];
-    a[4] = L.L40 * du[0] + L.L41 * du[1] + L.L42 * du[2] + L.L43 * du[3] + L.L44 * du[4] + s5 + s6;
-    a[6] = L.L60 * du[0] + L.L61 * du[1] + L.L62 * du[2] + L.L63 * du[3] + L.L64 * du[4] +
-           L.L05 * du[5] + L.L06 * du[6];
+    for (int n = 0; n < 7; n++) f[n] = 0.5 * (fl[n] + fr[n]);
   }
-  RoeR R;
-  roe_right(S, r.v1, r.v2, r.v3, r.x, eos, R);
+  const double du[7] = {Ur.d - Ul.d, Ur.mx - Ul.mx, Ur.my - Ul.my, Ur.mz - Ul.mz,
+                        Ur.e - Ul.e,  Ur.by - Ul.by, Ur.bz - Ul.bz};
+  double ui[7] = {Ul.d, Ul.mx, Ul.my, Ul.mz, Ul.e, Ul.by, Ul.bz};
+  RoeVecScalars V;
+  roe_vec_scalars(S, r.droe, r.v2, r.v3, eos, V);
+  double a[7];
+  roe_project(S, V, r.v1, r.v2, r.v3, r.b2, r.b3, r.x, eos, du, a);
   const double ev[7] = {S.ev0, S.ev1, S.ev2, S.ev3, S.ev4, S.ev5, S.ev6};
-  // intermediate states (flux_roe.c:212-237)
-  {
-    double ui[7] = {Ul.d, Ul.mx, Ul.my, Ul.mz, Ul.e, Ul.by, Ul.bz};
-    int hlle_flag = inter_state<0>(R, a, ev, Bxi, ui);
-    if (hlle_flag == 0) hlle_flag = inter_state<1>(R, a, ev, Bxi, ui);
-    if (hlle_flag == 0) hlle_flag = inter_state<2>(R, a, ev, Bxi, ui);
-    if (hlle_flag == 0) hlle_flag = inter_state<3>(R, a, ev, Bxi, ui);
-    if (hlle_flag == 0) hlle_flag = inter_state<4>(R, a, ev, Bxi, ui);
-    if (hlle_flag == 0) hlle_flag = inter_state<5>(R, a, ev, Bxi, ui);
-    if (hlle_flag != 0) return hlle_from_roe(Ul, Ur, Wl, Wr, Bxi, r, S.ev0, S.ev6, eos);
-  }
   double coeff[7];
 #pragma unroll
   for (int m = 0; m < 7; m++) coeff[m] = 0.5 * dmax(fabs(ev[m]), 0.0) * a[m];
-  const double* fl = &Fl.d;
-  const double* fr = &Fr.d;
-  double f[7];
-#pragma unroll
-  for (int n = 0; n < 7; n++) f[n] = 0.5 * (fl[n] + fr[n]);
-  // f[n] -= sum_m coeff[m] * R[n][m], m ascending (flux_roe.c:251-256)
-  f[0] = f[0] - coeff[0] * R.R00 - coeff[2] * R.R02 - coeff[3] - coeff[4] * R.R02 - coeff[6] * R.R00;
-  f[1] = f[1] - coeff[0] * R.R10 - coeff[2] * R.R12 - coeff[3] * R.R13 - coeff[4] * R.R14 -
-         coeff[6] * R.R16;
-  f[2] = f[2] - coeff[0] * R.R20 - coeff[1] * R.R21 - coeff[2] * R.R22 - coeff[3] * R.R23 -
-         coeff[4] * R.R24 - coeff[5] * R.R25 - coeff[6] * R.R26;
-  f[3] = f[3] - coeff[0] * R.R30 - coeff[1] * R.R31 - coeff[2] * R.R32 - coeff[3] * R.R33 -
-         coeff[4] * R.R34 - coeff[5] * R.R35 - coeff[6] * R.R36;
-  f[4] = f[4] - coeff[0] * R.R40 - coeff[1] * R.R41 - coeff[2] * R.R42 - coeff[3] * R.R43 -
-         coeff[4] * R.R44 - coeff[5] * (-R.R41) - coeff[6] * R.R46;
-  f[5] = f[5] - coeff[0] * R.R50 - coeff[1] * R.R51 - coeff[2] * R.R52 - coeff[4] * R.R52 -
-         coeff[5] * R.R51 - coeff[6] * R.R50;
-  f[6] = f[6] - coeff[0] * R.R60 - coeff[1] * R.R61 - coeff[2] * R.R62 - coeff[4] * R.R62 -
-         coeff[5] * R.R61 - coeff[6] * R.R60;
+  bool bad = false;
+  roe_column_pass<0>(S, V, r.v1, r.v2, r.v3, r.x, eos, a, coeff, ev, Bxi, ui, f, bad);
+  roe_column_pass<1>(S, V, r.v1, r.v2, r.v3, r.x, eos, a, coeff, ev, Bxi, ui, f, bad);
+  roe_column_pass<2>(S, V, r.v1, r.v2, r.v3, r.x, eos, a, coeff, ev, Bxi, ui, f, bad);
+  roe_column_pass<3>(S, V, r.v1, r.v2, r.v3, r.x, eos, a, coeff, ev, Bxi, ui, f, bad);
+  roe_column_pass<4>(S, V, r.v1, r.v2, r.v3, r.x, eos, a, coeff, ev, Bxi, ui, f, bad);
+  roe_column_pass<5>(S, V, r.v1, r.v2, r.v3, r.x, eos, a, coeff, ev, Bxi, ui, f, bad);
+  roe_column_pass<6>(S, V, r.v1, r.v2, r.v3, r.x, eos, a, coeff, ev, Bxi, ui, f, bad);
+  if (bad) {
+    Cons7 ul, ur;
+    Prim7 wl, wr;
+    double bx;
+    reload(ul, ur, wl, wr, bx);
+    return flux_hlle(ul, ur, wl, wr, bx, eos);
+  }
   return Cons7{f[0], f[1], f[2], f[3], f[4], f[5], f[6]};
 }
 
+ATHMI_HD Cons7 flux_roe(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl, const Prim7& Wr,
+                        double Bxi, const EosC& eos) {
+  return flux_roe_reload(Ul, Ur, Wl, Wr, Bxi, eos,
+                         [&](Cons7& ul, Cons7& ur, Prim7& wl, Prim7& wr, double& bx) {
+                           ul = Ul; ur = Ur; wl = Wl; wr = Wr; bx = Bxi;
+                         });
+}
+
 enum { kFluxRoe = 0, kFluxHlle = 1 };
+
+// Riemann solver of a grid; the Roe solver rebuilds its inputs for the HLLE
+// fallback through `reload` (see flux_roe_reload)
+template <class Reload>
+ATHMI_HD Cons7 riemann_reload(int kind, const Cons7& Ul, const Cons7& Ur, const Prim7& Wl,
+                              const Prim7& Wr, double Bxi, const EosC& eos, Reload&& reload) {
+  if (kind == kFluxHlle) return flux_hlle(Ul, Ur, Wl, Wr, Bxi, eos);
+  return flux_roe_reload(Ul, Ur, Wl, Wr, Bxi, eos, reload);
+}
+
 
 ATHMI_HD Cons7 riemann(int kind, const Cons7& Ul, const Cons7& Ur, const Prim7& Wl,
                        const Prim7& Wr, double Bxi, const EosC& eos) {

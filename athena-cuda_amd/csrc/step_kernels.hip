@@ -48,14 +48,37 @@ __device__ __forceinline__ void lds_put(double (*s)[kX1Block], int t, const Prim
   s[0][t] = w.d; s[1][t] = w.vx; s[2][t] = w.vy; s[3][t] = w.vz;
   s[4][t] = w.p; s[5][t] = w.by; s[6][t] = w.bz;
 }
-__device__ __forceinline__ void put7(double* base, size_t ps, size_t k, const Cons7& u) {
-  base[0 * ps + k] = u.d; base[1 * ps + k] = u.mx; base[2 * ps + k] = u.my;
-  base[3 * ps + k] = u.mz; base[4 * ps + k] = u.e; base[5 * ps + k] = u.by;
-  base[6 * ps + k] = u.bz;
+// Global memory through a uniform plane base and a 32-bit element index: the
+// hardware's saddr + voffset form, so all planes accessed at one cell share a
+// single offset register (a plane holds < 2^29 doubles).
+typedef __attribute__((address_space(1))) const char gconst_bytes;
+typedef __attribute__((address_space(1))) char gbytes;
+__device__ __forceinline__ double LD(const double* plane, uint32_t k) {
+  return *(__attribute__((address_space(1))) const double*)((gconst_bytes*)plane + k * 8u);
 }
-__device__ __forceinline__ Cons7 get7(const double* base, size_t ps, size_t k) {
-  return Cons7{base[0 * ps + k], base[1 * ps + k], base[2 * ps + k], base[3 * ps + k],
-               base[4 * ps + k], base[5 * ps + k], base[6 * ps + k]};
+__device__ __forceinline__ void ST(double* plane, uint32_t k, double v) {
+  *(__attribute__((address_space(1))) double*)((gbytes*)plane + k * 8u) = v;
+}
+
+__device__ __forceinline__ Prim7 lds_prim64(double (*s)[kX2Block], int t) {
+  return Prim7{s[0][t], s[1][t], s[2][t], s[3][t], s[4][t], s[5][t], s[6][t]};
+}
+__device__ __forceinline__ void lds_put64(double (*s)[kX2Block], int t, const Prim7& w) {
+  s[0][t] = w.d; s[1][t] = w.vx; s[2][t] = w.vy; s[3][t] = w.vz;
+  s[4][t] = w.p; s[5][t] = w.by; s[6][t] = w.bz;
+}
+// Keeps the compiler from forwarding earlier loads into a recomputation
+// (emits no instruction)
+__device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
+
+__device__ __forceinline__ void put7(double* base, size_t ps, uint32_t k, const Cons7& u) {
+  ST(base + 0 * ps, k, u.d); ST(base + 1 * ps, k, u.mx); ST(base + 2 * ps, k, u.my);
+  ST(base + 3 * ps, k, u.mz); ST(base + 4 * ps, k, u.e); ST(base + 5 * ps, k, u.by);
+  ST(base + 6 * ps, k, u.bz);
+}
+__device__ __forceinline__ Cons7 get7(const double* base, size_t ps, uint32_t k) {
+  return Cons7{LD(base + 0 * ps, k), LD(base + 1 * ps, k), LD(base + 2 * ps, k), LD(base + 3 * ps, k),
+               LD(base + 4 * ps, k), LD(base + 5 * ps, k), LD(base + 6 * ps, k)};
 }
 
 __device__ __forceinline__ double block_max256(double v, double* red) {
@@ -104,18 +127,18 @@ __global__ void __launch_bounds__(kX1Block) k_pred_x1(const Inst* __restrict__ i
     const double* __restrict__ st = g.state[clk->cur];
     double* __restrict__ sc = g.scratch;
     const size_t ps = g.ps;
-    const size_t rk = (size_t)j * g.pitch;
+    const uint32_t rk = (uint32_t)(j * g.pitch);
     double bxc = 0.0;
     if (cell >= g.is - 3 && cell <= g.ie + 3) {
-      const size_t k = rk + cell;
-      const double d = st[F_D * ps + k], m1 = st[F_M1 * ps + k], m2 = st[F_M2 * ps + k];
-      const double b1c = st[F_B1C * ps + k], b2c = st[F_B2C * ps + k];
-      const Cons7 u{d, m1, m2, st[F_M3 * ps + k], st[F_E * ps + k], b2c, st[F_B3C * ps + k]};
+      const uint32_t k = rk + (uint32_t)cell;
+      const double d = LD(st + F_D * ps, k), m1 = LD(st + F_M1 * ps, k), m2 = LD(st + F_M2 * ps, k);
+      const double b1c = LD(st + F_B1C * ps, k), b2c = LD(st + F_B2C * ps, k);
+      const Cons7 u{d, m1, m2, LD(st + F_M3 * ps, k), LD(st + F_E * ps, k), b2c, LD(st + F_B3C * ps, k)};
       bxc = b1c;
       lds_put(sw, tid, cons_to_prim(u, bxc, eos));
       // Step 3 (:249-255): emf3 at t^n over [jl,ju] x [il,iu]
       if (cell >= g.is - 2 && cell <= g.ie + 2 && tid >= 1 && tid <= kX1Block - 2)
-        sc[S_ECC * ps + k] = (b1c * m2 - b2c * m1) / d;
+        ST(sc + S_ECC * ps, k, (b1c * m2 - b2c * m1) / d);
     }
     __syncthreads();
     if (tid >= 1 && tid <= kX1Block - 2 && cell >= g.is - 2 && cell <= g.ie + 2) {
@@ -127,20 +150,29 @@ __global__ void __launch_bounds__(kX1Block) k_pred_x1(const Inst* __restrict__ i
     }
     __syncthreads();
     if (tid >= 2 && tid <= kX1Block - 2 && cell <= g.ie + 2) {
-      const size_t k = rk + cell;
-      Prim7 wl = lds_prim(swl, tid), wr = lds_prim(swr, tid);
-      const double bm = st[F_B1I * ps + k - 1], b0 = st[F_B1I * ps + k],
-                   bp = st[F_B1I * ps + k + 1];
-      // MHD source terms for 1/2 dt (:156-164)
-      double src = (st[F_M2 * ps + k - 1] / st[F_D * ps + k - 1]) * (b0 - bm) / g.dx1;
-      wl.by = wl.by + hdt * src;
-      src = (st[F_M2 * ps + k] / st[F_D * ps + k]) * (bp - b0) / g.dx1;
-      wr.by = wr.by + hdt * src;
-      const Cons7 ul = prim_to_cons(wl, b0, eos);
-      const Cons7 ur = prim_to_cons(wr, b0, eos);
+      const uint32_t k = rk + (uint32_t)cell;
+      // face states: PLM output + MHD source terms for 1/2 dt (:156-164) +
+      // prim->cons (:175-176); rebuilt from LDS and memory for the HLLE fallback
+      auto build = [&](Cons7& ul, Cons7& ur, Prim7& wl, Prim7& wr, double& b0) {
+        compiler_fence();
+        wl = lds_prim(swl, tid);
+        wr = lds_prim(swr, tid);
+        const double bm = LD(st + F_B1I * ps, k - 1), bp = LD(st + F_B1I * ps, k + 1);
+        b0 = LD(st + F_B1I * ps, k);
+        double src = (LD(st + F_M2 * ps, k - 1) / LD(st + F_D * ps, k - 1)) * (b0 - bm) / g.dx1;
+        wl.by = wl.by + hdt * src;
+        src = (LD(st + F_M2 * ps, k) / LD(st + F_D * ps, k)) * (bp - b0) / g.dx1;
+        wr.by = wr.by + hdt * src;
+        ul = prim_to_cons(wl, b0, eos);
+        ur = prim_to_cons(wr, b0, eos);
+      };
+      Cons7 ul, ur;
+      Prim7 wl, wr;
+      double b0;
+      build(ul, ur, wl, wr, b0);
       put7(sc + S_UL1 * ps, ps, k, ul);
       put7(sc + S_UR1 * ps, ps, k, ur);
-      put7(sc + S_F1 * ps, ps, k, riemann(g.flux, ul, ur, wl, wr, b0, eos));
+      put7(sc + S_F1 * ps, ps, k, riemann_reload(g.flux, ul, ur, wl, wr, b0, eos, build));
     }
     __syncthreads();
   }
@@ -153,6 +185,8 @@ __global__ void __launch_bounds__(kX1Block) k_pred_x1(const Inst* __restrict__ i
 // the three-cell PLM stencil and the pending left state in registers.
 __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ insts, int ninst,
                                                       int ntiles, int seg_len) {
+  __shared__ double sxl[7][kX2Block];
+  __shared__ double sxr[7][kX2Block];
   const int lane = threadIdx.x;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Inst& g = insts[find_inst(insts, ninst, K_PRED_X2, t)];
@@ -174,10 +208,10 @@ __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ i
     const size_t ps = g.ps;
     const int pitch = g.pitch;
     auto prim_at = [&](int jj, double& bx) -> Prim7 {
-      const size_t k = (size_t)jj * pitch + i;
-      const Cons7 u{st[F_D * ps + k], st[F_M2 * ps + k], st[F_M3 * ps + k], st[F_M1 * ps + k],
-                    st[F_E * ps + k], st[F_B3C * ps + k], st[F_B1C * ps + k]};
-      bx = st[F_B2C * ps + k];
+      const uint32_t k = (uint32_t)(jj * pitch + i);
+      const Cons7 u{LD(st + F_D * ps, k), LD(st + F_M2 * ps, k), LD(st + F_M3 * ps, k), LD(st + F_M1 * ps, k),
+                    LD(st + F_E * ps, k), LD(st + F_B3C * ps, k), LD(st + F_B1C * ps, k)};
+      bx = LD(st + F_B2C * ps, k);
       return cons_to_prim(u, bx, eos);
     };
     double bxa, bxb, bxc;
@@ -193,19 +227,31 @@ __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ i
       wc = prim_at(j + 1, bxc);
       Prim7 wl_next, wr_cur;
       plm_cell(wa, wb, wc, bxb, dtodx2, eos, wl_next, wr_cur);
-      const size_t k = (size_t)j * pitch + i;
-      const double bm = st[F_B2I * ps + k - pitch], b0 = st[F_B2I * ps + k],
-                   bp = st[F_B2I * ps + k + pitch];
-      // MHD source terms for 1/2 dt (:216-224)
-      double src = (st[F_M1 * ps + k - pitch] / st[F_D * ps + k - pitch]) * (b0 - bm) / g.dx2;
-      wl_cur.bz = wl_cur.bz + hdt * src;
-      src = (st[F_M1 * ps + k] / st[F_D * ps + k]) * (bp - b0) / g.dx2;
-      wr_cur.bz = wr_cur.bz + hdt * src;
-      const Cons7 ul = prim_to_cons(wl_cur, b0, eos);
-      const Cons7 ur = prim_to_cons(wr_cur, b0, eos);
+      const uint32_t k = (uint32_t)(j * pitch + i);
+      lds_put64(sxl, lane, wl_cur);
+      lds_put64(sxr, lane, wr_cur);
+      // face states: PLM output + MHD source terms for 1/2 dt (:216-224) +
+      // prim->cons (:235-236); rebuilt from LDS and memory for the HLLE fallback
+      auto build = [&](Cons7& ul, Cons7& ur, Prim7& wl, Prim7& wr, double& b0) {
+        compiler_fence();
+        wl = lds_prim64(sxl, lane);
+        wr = lds_prim64(sxr, lane);
+        const double bm = LD(st + F_B2I * ps, k - pitch), bp = LD(st + F_B2I * ps, k + pitch);
+        b0 = LD(st + F_B2I * ps, k);
+        double src = (LD(st + F_M1 * ps, k - pitch) / LD(st + F_D * ps, k - pitch)) * (b0 - bm) / g.dx2;
+        wl.bz = wl.bz + hdt * src;
+        src = (LD(st + F_M1 * ps, k) / LD(st + F_D * ps, k)) * (bp - b0) / g.dx2;
+        wr.bz = wr.bz + hdt * src;
+        ul = prim_to_cons(wl, b0, eos);
+        ur = prim_to_cons(wr, b0, eos);
+      };
+      Cons7 ul, ur;
+      Prim7 wl, wr;
+      double b0;
+      build(ul, ur, wl, wr, b0);
       put7(sc + S_UL2 * ps, ps, k, ul);
       put7(sc + S_UR2 * ps, ps, k, ur);
-      put7(sc + S_F2 * ps, ps, k, riemann(g.flux, ul, ur, wl_cur, wr_cur, b0, eos));
+      put7(sc + S_F2 * ps, ps, k, riemann_reload(g.flux, ul, ur, wl, wr, b0, eos, build));
       wl_cur = wl_next;
     }
   }
@@ -236,14 +282,14 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_corner(const Inst* __restric
     const double* F1 = sc + (PRED ? S_F1 : S_F1C) * ps;
     const double* F2 = sc + (PRED ? S_F2 : S_F2C) * ps;
     const double* ecc = sc + (PRED ? S_ECC : S_ECH) * ps;
-    const size_t k = (size_t)j * pitch + i;
-    const size_t kjm = k - pitch, kim = k - 1, kjmim = k - pitch - 1;
+    const uint32_t k = (uint32_t)(j * pitch + i);
+    const uint32_t kjm = k - pitch, kim = k - 1, kjmim = k - pitch - 1;
     // component offsets within a Cons1D group: d = 0, By = 5, Bz = 6
-    const double f1d_jm = F1[kjm], f1d = F1[k];
-    const double f1by_jm = F1[5 * ps + kjm], f1by = F1[5 * ps + k];
-    const double f2d_im = F2[kim], f2d = F2[k];
-    const double f2bz_im = F2[6 * ps + kim], f2bz = F2[6 * ps + k];
-    const double e_jmim = ecc[kjmim], e_jm = ecc[kjm], e_im = ecc[kim], e_00 = ecc[k];
+    const double f1d_jm = LD(F1, kjm), f1d = LD(F1, k);
+    const double f1by_jm = LD(F1 + 5 * ps, kjm), f1by = LD(F1 + 5 * ps, k);
+    const double f2d_im = LD(F2, kim), f2d = LD(F2, k);
+    const double f2bz_im = LD(F2 + 6 * ps, kim), f2bz = LD(F2 + 6 * ps, k);
+    const double e_jmim = LD(ecc, kjmim), e_jm = LD(ecc, kjm), e_im = LD(ecc, kim), e_00 = LD(ecc, k);
     double emf_l1, emf_r1, emf_l2, emf_r2;
     if (f1d_jm > 0.0) emf_l2 = -f1by_jm + (f2bz_im - e_jmim);
     else if (f1d_jm < 0.0) emf_l2 = -f1by_jm + (f2bz - e_jm);
@@ -257,7 +303,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_corner(const Inst* __restric
     if (f2d > 0.0) emf_r1 = f2bz + (-f1by_jm - e_jm);
     else if (f2d < 0.0) emf_r1 = f2bz + (-f1by - e_00);
     else emf_r1 = f2bz + 0.5 * (-f1by_jm - e_jm - f1by - e_00);
-    sc[S_EMF * ps + k] = 0.25 * (emf_l1 + emf_r1 + emf_l2 + emf_r2);
+    ST(sc + S_EMF * ps, k, 0.25 * (emf_l1 + emf_r1 + emf_l2 + emf_r2));
   }
 }
 
@@ -289,107 +335,125 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_corr(const Inst* __restrict_
     double* __restrict__ sc = g.scratch;
     const size_t ps = g.ps;
     const int pitch = g.pitch;
-    const size_t k = (size_t)j * pitch + i;
+    const uint32_t k = (uint32_t)(j * pitch + i);
     const double* emf = sc + S_EMF * ps;
     const double* B1i = st + F_B1I * ps;
     const double* B2i = st + F_B2I * ps;
-    auto b1h = [&](size_t kk) { return B1i[kk] - hdtodx2 * (emf[kk + pitch] - emf[kk]); };
-    auto b2h = [&](size_t kk) { return B2i[kk] + hdtodx1 * (emf[kk + 1] - emf[kk]); };
+    auto b1h = [&](uint32_t kk) { return LD(B1i, kk) - hdtodx2 * (LD(emf, kk + pitch) - LD(emf, kk)); };
+    auto b2h = [&](uint32_t kk) { return LD(B2i, kk) + hdtodx1 * (LD(emf, kk + 1) - LD(emf, kk)); };
     const double* F1 = sc + S_F1 * ps;
     const double* F2 = sc + S_F2 * ps;
     if (i >= g.is) {
-      Cons7 L = get7(sc + S_UL1 * ps, ps, k);
-      Cons7 R = get7(sc + S_UR1 * ps, ps, k);
-      {  // Step 5a: x2 fluxes at (j+1,i-1),(j,i-1) for L and (j+1,i),(j,i) for R
-        const size_t a1 = k + pitch - 1, a0 = k - 1, c1 = k + pitch, c0 = k;
-        L.d = L.d - hdtodx2 * (F2[a1] - F2[a0]);
-        L.mx = L.mx - hdtodx2 * (F2[3 * ps + a1] - F2[3 * ps + a0]);
-        L.my = L.my - hdtodx2 * (F2[1 * ps + a1] - F2[1 * ps + a0]);
-        L.mz = L.mz - hdtodx2 * (F2[2 * ps + a1] - F2[2 * ps + a0]);
-        L.e = L.e - hdtodx2 * (F2[4 * ps + a1] - F2[4 * ps + a0]);
-        L.bz = L.bz - hdtodx2 * (F2[5 * ps + a1] - F2[5 * ps + a0]);
-        R.d = R.d - hdtodx2 * (F2[c1] - F2[c0]);
-        R.mx = R.mx - hdtodx2 * (F2[3 * ps + c1] - F2[3 * ps + c0]);
-        R.my = R.my - hdtodx2 * (F2[1 * ps + c1] - F2[1 * ps + c0]);
-        R.mz = R.mz - hdtodx2 * (F2[2 * ps + c1] - F2[2 * ps + c0]);
-        R.e = R.e - hdtodx2 * (F2[4 * ps + c1] - F2[4 * ps + c0]);
-        R.bz = R.bz - hdtodx2 * (F2[5 * ps + c1] - F2[5 * ps + c0]);
-      }
-      {  // Step 5b: MHD source terms with the cell-centred field of cells i-1, i
-        double dbx = B1i[k] - B1i[k - 1];
-        double b1 = st[F_B1C * ps + k - 1], b2 = st[F_B2C * ps + k - 1], b3 = st[F_B3C * ps + k - 1];
-        double v3 = st[F_M3 * ps + k - 1] / st[F_D * ps + k - 1];
-        L.mx = L.mx + hdtodx1 * b1 * dbx;
-        L.my = L.my + hdtodx1 * b2 * dbx;
-        L.mz = L.mz + hdtodx1 * b3 * dbx;
-        L.bz = L.bz + hdtodx1 * v3 * dbx;
-        L.e = L.e + hdtodx1 * b3 * v3 * dbx;
-        dbx = B1i[k + 1] - B1i[k];
-        b1 = st[F_B1C * ps + k]; b2 = st[F_B2C * ps + k]; b3 = st[F_B3C * ps + k];
-        v3 = st[F_M3 * ps + k] / st[F_D * ps + k];
-        R.mx = R.mx + hdtodx1 * b1 * dbx;
-        R.my = R.my + hdtodx1 * b2 * dbx;
-        R.mz = R.mz + hdtodx1 * b3 * dbx;
-        R.bz = R.bz + hdtodx1 * v3 * dbx;
-        R.e = R.e + hdtodx1 * b3 * v3 * dbx;
-      }
-      const double bx = b1h(k);
-      const Prim7 wl = cons_to_prim(L, bx, eos);
-      const Prim7 wr = cons_to_prim(R, bx, eos);
-      put7(sc + S_F1C * ps, ps, k, riemann(g.flux, L, R, wl, wr, bx, eos));
+      // corrected x1-face states (Steps 5a, 5b) at the half-step face field;
+      // rebuilt from memory for the HLLE fallback
+      auto build = [&](Cons7& L, Cons7& R, Prim7& wl, Prim7& wr, double& bx) {
+        compiler_fence();
+        L = get7(sc + S_UL1 * ps, ps, k);
+        R = get7(sc + S_UR1 * ps, ps, k);
+        {  // Step 5a: x2 fluxes at (j+1,i-1),(j,i-1) for L and (j+1,i),(j,i) for R
+          const uint32_t a1 = k + pitch - 1, a0 = k - 1, c1 = k + pitch, c0 = k;
+          L.d = L.d - hdtodx2 * (LD(F2, a1) - LD(F2, a0));
+          L.mx = L.mx - hdtodx2 * (LD(F2 + 3 * ps, a1) - LD(F2 + 3 * ps, a0));
+          L.my = L.my - hdtodx2 * (LD(F2 + 1 * ps, a1) - LD(F2 + 1 * ps, a0));
+          L.mz = L.mz - hdtodx2 * (LD(F2 + 2 * ps, a1) - LD(F2 + 2 * ps, a0));
+          L.e = L.e - hdtodx2 * (LD(F2 + 4 * ps, a1) - LD(F2 + 4 * ps, a0));
+          L.bz = L.bz - hdtodx2 * (LD(F2 + 5 * ps, a1) - LD(F2 + 5 * ps, a0));
+          R.d = R.d - hdtodx2 * (LD(F2, c1) - LD(F2, c0));
+          R.mx = R.mx - hdtodx2 * (LD(F2 + 3 * ps, c1) - LD(F2 + 3 * ps, c0));
+          R.my = R.my - hdtodx2 * (LD(F2 + 1 * ps, c1) - LD(F2 + 1 * ps, c0));
+          R.mz = R.mz - hdtodx2 * (LD(F2 + 2 * ps, c1) - LD(F2 + 2 * ps, c0));
+          R.e = R.e - hdtodx2 * (LD(F2 + 4 * ps, c1) - LD(F2 + 4 * ps, c0));
+          R.bz = R.bz - hdtodx2 * (LD(F2 + 5 * ps, c1) - LD(F2 + 5 * ps, c0));
+        }
+        {  // Step 5b: MHD source terms with the cell-centred field of cells i-1, i
+          double dbx = LD(B1i, k) - LD(B1i, k - 1);
+          double b1 = LD(st + F_B1C * ps, k - 1), b2 = LD(st + F_B2C * ps, k - 1);
+          double b3 = LD(st + F_B3C * ps, k - 1);
+          double v3 = LD(st + F_M3 * ps, k - 1) / LD(st + F_D * ps, k - 1);
+          L.mx = L.mx + hdtodx1 * b1 * dbx;
+          L.my = L.my + hdtodx1 * b2 * dbx;
+          L.mz = L.mz + hdtodx1 * b3 * dbx;
+          L.bz = L.bz + hdtodx1 * v3 * dbx;
+          L.e = L.e + hdtodx1 * b3 * v3 * dbx;
+          dbx = LD(B1i, k + 1) - LD(B1i, k);
+          b1 = LD(st + F_B1C * ps, k); b2 = LD(st + F_B2C * ps, k); b3 = LD(st + F_B3C * ps, k);
+          v3 = LD(st + F_M3 * ps, k) / LD(st + F_D * ps, k);
+          R.mx = R.mx + hdtodx1 * b1 * dbx;
+          R.my = R.my + hdtodx1 * b2 * dbx;
+          R.mz = R.mz + hdtodx1 * b3 * dbx;
+          R.bz = R.bz + hdtodx1 * v3 * dbx;
+          R.e = R.e + hdtodx1 * b3 * v3 * dbx;
+        }
+        bx = b1h(k);
+        wl = cons_to_prim(L, bx, eos);
+        wr = cons_to_prim(R, bx, eos);
+      };
+      Cons7 L, R;
+      Prim7 wl, wr;
+      double bx;
+      build(L, R, wl, wr, bx);
+      put7(sc + S_F1C * ps, ps, k, riemann_reload(g.flux, L, R, wl, wr, bx, eos, build));
     }
     if (j >= g.js) {
-      Cons7 L = get7(sc + S_UL2 * ps, ps, k);
-      Cons7 R = get7(sc + S_UR2 * ps, ps, k);
-      {  // Step 6a: x1 fluxes at (j-1,i+1),(j-1,i) for L and (j,i+1),(j,i) for R
-        const size_t a1 = k - pitch + 1, a0 = k - pitch, c1 = k + 1, c0 = k;
-        L.d = L.d - hdtodx1 * (F1[a1] - F1[a0]);
-        L.mx = L.mx - hdtodx1 * (F1[2 * ps + a1] - F1[2 * ps + a0]);
-        L.my = L.my - hdtodx1 * (F1[3 * ps + a1] - F1[3 * ps + a0]);
-        L.mz = L.mz - hdtodx1 * (F1[1 * ps + a1] - F1[1 * ps + a0]);
-        L.e = L.e - hdtodx1 * (F1[4 * ps + a1] - F1[4 * ps + a0]);
-        L.by = L.by - hdtodx1 * (F1[6 * ps + a1] - F1[6 * ps + a0]);
-        R.d = R.d - hdtodx1 * (F1[c1] - F1[c0]);
-        R.mx = R.mx - hdtodx1 * (F1[2 * ps + c1] - F1[2 * ps + c0]);
-        R.my = R.my - hdtodx1 * (F1[3 * ps + c1] - F1[3 * ps + c0]);
-        R.mz = R.mz - hdtodx1 * (F1[1 * ps + c1] - F1[1 * ps + c0]);
-        R.e = R.e - hdtodx1 * (F1[4 * ps + c1] - F1[4 * ps + c0]);
-        R.by = R.by - hdtodx1 * (F1[6 * ps + c1] - F1[6 * ps + c0]);
-      }
-      {  // Step 6b: MHD source terms with the cell-centred field of cells j-1, j
-        const size_t km = k - pitch;
-        double dby = B2i[k] - B2i[km];
-        double b1 = st[F_B1C * ps + km], b2 = st[F_B2C * ps + km], b3 = st[F_B3C * ps + km];
-        double v3 = st[F_M3 * ps + km] / st[F_D * ps + km];
-        L.mz = L.mz + hdtodx2 * b1 * dby;
-        L.mx = L.mx + hdtodx2 * b2 * dby;
-        L.my = L.my + hdtodx2 * b3 * dby;
-        L.by = L.by + hdtodx2 * v3 * dby;
-        L.e = L.e + hdtodx2 * b3 * v3 * dby;
-        dby = B2i[k + pitch] - B2i[k];
-        b1 = st[F_B1C * ps + k]; b2 = st[F_B2C * ps + k]; b3 = st[F_B3C * ps + k];
-        v3 = st[F_M3 * ps + k] / st[F_D * ps + k];
-        R.mz = R.mz + hdtodx2 * b1 * dby;
-        R.mx = R.mx + hdtodx2 * b2 * dby;
-        R.my = R.my + hdtodx2 * b3 * dby;
-        R.by = R.by + hdtodx2 * v3 * dby;
-        R.e = R.e + hdtodx2 * b3 * v3 * dby;
-      }
-      const double bx = b2h(k);
-      const Prim7 wl = cons_to_prim(L, bx, eos);
-      const Prim7 wr = cons_to_prim(R, bx, eos);
-      put7(sc + S_F2C * ps, ps, k, riemann(g.flux, L, R, wl, wr, bx, eos));
+      // corrected x2-face states (Steps 6a, 6b) at the half-step face field
+      auto build = [&](Cons7& L, Cons7& R, Prim7& wl, Prim7& wr, double& bx) {
+        compiler_fence();
+        L = get7(sc + S_UL2 * ps, ps, k);
+        R = get7(sc + S_UR2 * ps, ps, k);
+        {  // Step 6a: x1 fluxes at (j-1,i+1),(j-1,i) for L and (j,i+1),(j,i) for R
+          const uint32_t a1 = k - pitch + 1, a0 = k - pitch, c1 = k + 1, c0 = k;
+          L.d = L.d - hdtodx1 * (LD(F1, a1) - LD(F1, a0));
+          L.mx = L.mx - hdtodx1 * (LD(F1 + 2 * ps, a1) - LD(F1 + 2 * ps, a0));
+          L.my = L.my - hdtodx1 * (LD(F1 + 3 * ps, a1) - LD(F1 + 3 * ps, a0));
+          L.mz = L.mz - hdtodx1 * (LD(F1 + 1 * ps, a1) - LD(F1 + 1 * ps, a0));
+          L.e = L.e - hdtodx1 * (LD(F1 + 4 * ps, a1) - LD(F1 + 4 * ps, a0));
+          L.by = L.by - hdtodx1 * (LD(F1 + 6 * ps, a1) - LD(F1 + 6 * ps, a0));
+          R.d = R.d - hdtodx1 * (LD(F1, c1) - LD(F1, c0));
+          R.mx = R.mx - hdtodx1 * (LD(F1 + 2 * ps, c1) - LD(F1 + 2 * ps, c0));
+          R.my = R.my - hdtodx1 * (LD(F1 + 3 * ps, c1) - LD(F1 + 3 * ps, c0));
+          R.mz = R.mz - hdtodx1 * (LD(F1 + 1 * ps, c1) - LD(F1 + 1 * ps, c0));
+          R.e = R.e - hdtodx1 * (LD(F1 + 4 * ps, c1) - LD(F1 + 4 * ps, c0));
+          R.by = R.by - hdtodx1 * (LD(F1 + 6 * ps, c1) - LD(F1 + 6 * ps, c0));
+        }
+        {  // Step 6b: MHD source terms with the cell-centred field of cells j-1, j
+          const uint32_t km = k - pitch;
+          double dby = LD(B2i, k) - LD(B2i, km);
+          double b1 = LD(st + F_B1C * ps, km), b2 = LD(st + F_B2C * ps, km), b3 = LD(st + F_B3C * ps, km);
+          double v3 = LD(st + F_M3 * ps, km) / LD(st + F_D * ps, km);
+          L.mz = L.mz + hdtodx2 * b1 * dby;
+          L.mx = L.mx + hdtodx2 * b2 * dby;
+          L.my = L.my + hdtodx2 * b3 * dby;
+          L.by = L.by + hdtodx2 * v3 * dby;
+          L.e = L.e + hdtodx2 * b3 * v3 * dby;
+          dby = LD(B2i, k + pitch) - LD(B2i, k);
+          b1 = LD(st + F_B1C * ps, k); b2 = LD(st + F_B2C * ps, k); b3 = LD(st + F_B3C * ps, k);
+          v3 = LD(st + F_M3 * ps, k) / LD(st + F_D * ps, k);
+          R.mz = R.mz + hdtodx2 * b1 * dby;
+          R.mx = R.mx + hdtodx2 * b2 * dby;
+          R.my = R.my + hdtodx2 * b3 * dby;
+          R.by = R.by + hdtodx2 * v3 * dby;
+          R.e = R.e + hdtodx2 * b3 * v3 * dby;
+        }
+        bx = b2h(k);
+        wl = cons_to_prim(L, bx, eos);
+        wr = cons_to_prim(R, bx, eos);
+      };
+      Cons7 L, R;
+      Prim7 wl, wr;
+      double bx;
+      build(L, R, wl, wr, bx);
+      put7(sc + S_F2C * ps, ps, k, riemann_reload(g.flux, L, R, wl, wr, bx, eos, build));
     }
     // Step 7
-    const double dhalf = st[F_D * ps + k] - hdtodx1 * (F1[k + 1] - F1[k]) -
-                         hdtodx2 * (F2[k + pitch] - F2[k]);
-    const double m1 = st[F_M1 * ps + k] - hdtodx1 * (F1[1 * ps + k + 1] - F1[1 * ps + k]) -
-                      hdtodx2 * (F2[3 * ps + k + pitch] - F2[3 * ps + k]);
-    const double m2 = st[F_M2 * ps + k] - hdtodx1 * (F1[2 * ps + k + 1] - F1[2 * ps + k]) -
-                      hdtodx2 * (F2[1 * ps + k + pitch] - F2[1 * ps + k]);
+    const double dhalf = LD(st + F_D * ps, k) - hdtodx1 * (LD(F1, k + 1) - LD(F1, k)) -
+                         hdtodx2 * (LD(F2, k + pitch) - LD(F2, k));
+    const double m1 = LD(st + F_M1 * ps, k) - hdtodx1 * (LD(F1 + 1 * ps, k + 1) - LD(F1 + 1 * ps, k)) -
+                      hdtodx2 * (LD(F2 + 3 * ps, k + pitch) - LD(F2 + 3 * ps, k));
+    const double m2 = LD(st + F_M2 * ps, k) - hdtodx1 * (LD(F1 + 2 * ps, k + 1) - LD(F1 + 2 * ps, k)) -
+                      hdtodx2 * (LD(F2 + 1 * ps, k + pitch) - LD(F2 + 1 * ps, k));
     const double b1c = 0.5 * (b1h(k) + b1h(k + 1));
     const double b2c = 0.5 * (b2h(k) + b2h(k + pitch));
-    sc[S_ECH * ps + k] = (b1c * m2 - b2c * m1) / dhalf;
+    ST(sc + S_ECH * ps, k, (b1c * m2 - b2c * m1) / dhalf);
   }
 }
 
@@ -433,48 +497,48 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict
       const double* __restrict__ sc = g.scratch;
       const size_t ps = g.ps;
       const int pitch = g.pitch;
-      const size_t k = (size_t)j * pitch + i;
+      const uint32_t k = (uint32_t)(j * pitch + i);
       const double* emf = sc + S_EMF * ps;
-      const double e00 = emf[k], e10 = emf[k + pitch], e01 = emf[k + 1];
+      const double e00 = LD(emf, k), e10 = LD(emf, k + pitch), e01 = LD(emf, k + 1);
       double b1n = 0.0, b2n = 0.0;
       if (j <= g.je) {
-        b1n = st[F_B1I * ps + k] - dtodx2 * (e10 - e00);
-        out[F_B1I * ps + k] = b1n;
+        b1n = LD(st + F_B1I * ps, k) - dtodx2 * (e10 - e00);
+        ST(out + F_B1I * ps, k, b1n);
       }
       if (i <= g.ie) {
-        b2n = st[F_B2I * ps + k] + dtodx1 * (e01 - e00);
-        out[F_B2I * ps + k] = b2n;
+        b2n = LD(st + F_B2I * ps, k) + dtodx1 * (e01 - e00);
+        ST(out + F_B2I * ps, k, b2n);
       }
       if (j <= g.je && i <= g.ie) {
-        const double e11 = emf[k + pitch + 1];
-        const double b1p = st[F_B1I * ps + k + 1] - dtodx2 * (e11 - e01);
-        const double b2p = st[F_B2I * ps + k + pitch] + dtodx1 * (e11 - e10);
+        const double e11 = LD(emf, k + pitch + 1);
+        const double b1p = LD(st + F_B1I * ps, k + 1) - dtodx2 * (e11 - e01);
+        const double b2p = LD(st + F_B2I * ps, k + pitch) + dtodx1 * (e11 - e10);
         const double* F1 = sc + S_F1C * ps;
         const double* F2 = sc + S_F2C * ps;
-        double d = st[F_D * ps + k], M1 = st[F_M1 * ps + k], M2 = st[F_M2 * ps + k];
-        double M3 = st[F_M3 * ps + k], E = st[F_E * ps + k], B3c = st[F_B3C * ps + k];
-        d = d - dtodx1 * (F1[k + 1] - F1[k]);
-        M1 = M1 - dtodx1 * (F1[1 * ps + k + 1] - F1[1 * ps + k]);
-        M2 = M2 - dtodx1 * (F1[2 * ps + k + 1] - F1[2 * ps + k]);
-        M3 = M3 - dtodx1 * (F1[3 * ps + k + 1] - F1[3 * ps + k]);
-        E = E - dtodx1 * (F1[4 * ps + k + 1] - F1[4 * ps + k]);
-        B3c = B3c - dtodx1 * (F1[6 * ps + k + 1] - F1[6 * ps + k]);
-        const size_t kp = k + pitch;
-        d = d - dtodx2 * (F2[kp] - F2[k]);
-        M1 = M1 - dtodx2 * (F2[3 * ps + kp] - F2[3 * ps + k]);
-        M2 = M2 - dtodx2 * (F2[1 * ps + kp] - F2[1 * ps + k]);
-        M3 = M3 - dtodx2 * (F2[2 * ps + kp] - F2[2 * ps + k]);
-        E = E - dtodx2 * (F2[4 * ps + kp] - F2[4 * ps + k]);
-        B3c = B3c - dtodx2 * (F2[5 * ps + kp] - F2[5 * ps + k]);
-        out[F_D * ps + k] = d;
-        out[F_M1 * ps + k] = M1;
-        out[F_M2 * ps + k] = M2;
-        out[F_M3 * ps + k] = M3;
-        out[F_E * ps + k] = E;
-        out[F_B1C * ps + k] = 0.5 * (b1n + b1p);
-        out[F_B2C * ps + k] = 0.5 * (b2n + b2p);
-        out[F_B3C * ps + k] = B3c;
-        out[F_B3I * ps + k] = B3c;
+        double d = LD(st + F_D * ps, k), M1 = LD(st + F_M1 * ps, k), M2 = LD(st + F_M2 * ps, k);
+        double M3 = LD(st + F_M3 * ps, k), E = LD(st + F_E * ps, k), B3c = LD(st + F_B3C * ps, k);
+        d = d - dtodx1 * (LD(F1, k + 1) - LD(F1, k));
+        M1 = M1 - dtodx1 * (LD(F1 + 1 * ps, k + 1) - LD(F1 + 1 * ps, k));
+        M2 = M2 - dtodx1 * (LD(F1 + 2 * ps, k + 1) - LD(F1 + 2 * ps, k));
+        M3 = M3 - dtodx1 * (LD(F1 + 3 * ps, k + 1) - LD(F1 + 3 * ps, k));
+        E = E - dtodx1 * (LD(F1 + 4 * ps, k + 1) - LD(F1 + 4 * ps, k));
+        B3c = B3c - dtodx1 * (LD(F1 + 6 * ps, k + 1) - LD(F1 + 6 * ps, k));
+        const uint32_t kp = k + pitch;
+        d = d - dtodx2 * (LD(F2, kp) - LD(F2, k));
+        M1 = M1 - dtodx2 * (LD(F2 + 3 * ps, kp) - LD(F2 + 3 * ps, k));
+        M2 = M2 - dtodx2 * (LD(F2 + 1 * ps, kp) - LD(F2 + 1 * ps, k));
+        M3 = M3 - dtodx2 * (LD(F2 + 2 * ps, kp) - LD(F2 + 2 * ps, k));
+        E = E - dtodx2 * (LD(F2 + 4 * ps, kp) - LD(F2 + 4 * ps, k));
+        B3c = B3c - dtodx2 * (LD(F2 + 5 * ps, kp) - LD(F2 + 5 * ps, k));
+        ST(out + F_D * ps, k, d);
+        ST(out + F_M1 * ps, k, M1);
+        ST(out + F_M2 * ps, k, M2);
+        ST(out + F_M3 * ps, k, M3);
+        ST(out + F_E * ps, k, E);
+        ST(out + F_B1C * ps, k, 0.5 * (b1n + b1p));
+        ST(out + F_B2C * ps, k, 0.5 * (b2n + b2p));
+        ST(out + F_B3C * ps, k, B3c);
+        ST(out + F_B3I * ps, k, B3c);
         dti = inverse_dt(g, d, M1, M2, M3, E);
       }
     }
@@ -487,7 +551,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict
 // set_bvals (set_bvals.c:114-145): the x1 fills then the x2 fills compose into
 // one map from each ghost cell to the interior cell it copies, so a single
 // pass reads only non-ghost cells. Per plane, the exceptions of the reference
-// hold: B1i[ie+1] and B2i[je+1] are not boundary values (:630, :873, :1011);
+// hold: LD(B1i, ie+1) and LD(B2i, je+1) are not boundary values (:630, :873, :1011);
 // B2i's x1 fills cover rows js..je+1; reflection negates M1 across x1 and M2
 // across x2 (:312-545), outflow copies the edge cell (:553-786), periodic
 // wraps (:793-1031).
@@ -567,17 +631,17 @@ __global__ void __launch_bounds__(kGhostBlock) k_ghost(const Inst* __restrict__ 
     double* buf = g.state[which ? (clk->cur ^ 1) : clk->cur];
     const double* src = (which == 2) ? g.state[clk->cur] : buf;
     const size_t ps = g.ps;
-    const size_t k = (size_t)j * g.pitch + i;
+    const uint32_t k = (uint32_t)(j * g.pitch + i);
 #pragma unroll
     for (int f = 0; f < NSTATE; f++) {
       int sj, si;
       bool neg;
       if (ghost_src(g, f, j, i, sj, si, neg)) {
         if (which == 2) {
-          buf[f * ps + k] = src[f * ps + k];
+          ST(buf + f * ps, k, LD(src + f * ps, k));
         } else {
-          const double v = src[f * ps + (size_t)sj * g.pitch + si];
-          buf[f * ps + k] = neg ? -v : v;
+          const double v = LD(src + f * ps, (size_t)sj * g.pitch + si);
+          ST(buf + f * ps, k, neg ? -v : v);
         }
       }
     }
@@ -641,9 +705,10 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_dti(const Inst* __restrict__
     double dti = 0.0;
     if (i <= g.ie && j <= g.je) {
       const double* st = g.state[clk->cur];
-      const size_t ps = g.ps, k = (size_t)j * g.pitch + i;
-      dti = inverse_dt(g, st[F_D * ps + k], st[F_M1 * ps + k], st[F_M2 * ps + k],
-                       st[F_M3 * ps + k], st[F_E * ps + k]);
+      const size_t ps = g.ps;
+      const uint32_t k = (uint32_t)(j * g.pitch + i);
+      dti = inverse_dt(g, LD(st + F_D * ps, k), LD(st + F_M1 * ps, k), LD(st + F_M2 * ps, k),
+                       LD(st + F_M3 * ps, k), LD(st + F_E * ps, k));
     }
     const double m = block_max256(dti, red);
     if (threadIdx.x == 0 && threadIdx.y == 0) atomic_max_pos(&clk->dti_slot[t % kDtSlots], m);
@@ -681,9 +746,10 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_divb(const Inst* __restrict_
     double v = 0.0;
     if (i <= g.ie && j <= g.je) {
       const double* st = g.state[clk->cur];
-      const size_t ps = g.ps, k = (size_t)j * g.pitch + i;
-      v = fabs((st[F_B1I * ps + k + 1] - st[F_B1I * ps + k]) / g.dx1 +
-               (st[F_B2I * ps + k + g.pitch] - st[F_B2I * ps + k]) / g.dx2);
+      const size_t ps = g.ps;
+      const uint32_t k = (uint32_t)(j * g.pitch + i);
+      v = fabs((LD(st + F_B1I * ps, k + 1) - LD(st + F_B1I * ps, k)) / g.dx1 +
+               (LD(st + F_B2I * ps, k + g.pitch) - LD(st + F_B2I * ps, k)) / g.dx2);
     }
     const double m = block_max256(v, red);
     if (threadIdx.x == 0 && threadIdx.y == 0) atomic_max_pos(&clk->divb_bits, m);

@@ -110,6 +110,9 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device-to-device copy kernel whose HBM "
+                         "counters calibrate FETCH_SIZE/WRITE_SIZE under rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
@@ -172,6 +175,10 @@ def main():
                           "frac": step_bw / HBM_PEAK_GBS},
     }
     g.close()
+    if args.calibrate_copy:
+        L.athmi_copy_calibration.argtypes = [ctypes.c_size_t, ctypes.c_int]
+        pkg._check(L.athmi_copy_calibration(1 << 30, 3), "copy calibration")
+        out["copy_calibration_bytes"] = {"read": 1 << 30, "write": 1 << 30, "reps": 3}
     if not args.no_cpu:
         log("[bench] CPU baseline (reference CPU integrator, 1 core)")
         out["cpu_baseline"] = cpu_baseline()

@@ -111,6 +111,10 @@ int athmi_use_graph(athmi_grid *g, int on);
  * athmi_step_kernel_name(k). No reference counterpart: measurement hook. */
 int athmi_profile_steps(athmi_grid *g, int nsteps, double *avg_ms, int nslots);
 const char *athmi_step_kernel_name(int k);
+/* Copy nbytes between two fresh device buffers reps times with a plain
+ * streaming kernel (k_copy_calibration): the reference traffic that scales
+ * rocprofv3's FETCH_SIZE / WRITE_SIZE. No reference counterpart. */
+int athmi_copy_calibration(size_t nbytes, int reps);
 
 /* Device pointers of the current state of one instance: 11 planes
  * (d, M1, M2, M3, E, B1c, B2c, B3c, B1i, B2i, B3i), each [Nx2+8][pitch] */
