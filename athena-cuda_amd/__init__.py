@@ -65,6 +65,7 @@ SIGNATURES = {
     "athmi_sync": (_I, [_P]),
     "athmi_set_shape": (_I, [_P, _I, _I]),
     "athmi_use_graph": (_I, [_P, _I]),
+    "athmi_autotune": (_I, [_P, _I, _IP, _IP, _DP]),
     "athmi_device_state": (_I, [_P, _I, ctypes.POINTER(_P), _IP]),
     "athmi_pgen_blast": (_I, [_P, _I, _D, _D, _D, _D, _D, _D, _D]),
     "athmi_pgen_field_loop": (_I, [_P, _I, _D, _D, _D, _I]),
@@ -245,6 +246,14 @@ class Grid:
 
     def set_shape(self, x2_seg=32, persist=0):
         _check(lib().athmi_set_shape(self._h, x2_seg, persist), "set_shape")
+
+    def autotune(self, steps=2):
+        """Pick the fastest launch shape for this batch (state is restored).
+        Returns (x2_seg, persist, ms_per_step)."""
+        a, b, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().athmi_autotune(self._h, steps, ctypes.byref(a), ctypes.byref(b),
+                                    ctypes.byref(ms)), "autotune")
+        return a.value, b.value, ms.value
 
     def use_graph(self, on=True):
         _check(lib().athmi_use_graph(self._h, 1 if on else 0), "use_graph")

@@ -469,6 +469,84 @@ int athmi_set_shape(athmi_grid* g, int x2_seg, int persist) {
   return g->refresh_tiles();
 }
 
+// Launch-shape autotuner: every candidate (x2 marching depth x persistent
+// occupancy) runs the same `steps` main-loop steps from the same state, timed
+// with HIP events on the handle's stream through the step graph; the state
+// and clocks are restored after each candidate and the fastest shape is kept.
+int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, double* ms_per_step) {
+  if (!g || steps <= 0) return fail(ATHMI_EINVAL, "athmi_autotune: bad arguments");
+  static const int kSeg[] = {16, 32, 64, 128};
+  static const int kPersist[] = {0, 1, 2, 4, 8};
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  std::vector<Clock> hc(g->n);
+  HIP_TRY(hipMemcpy(hc.data(), g->dclk, sizeof(Clock) * g->n, hipMemcpyDeviceToHost));
+  size_t snap_doubles = 0;
+  for (const Inst& I : g->hinst) snap_doubles += NSTATE * I.ps;
+  double* snap = nullptr;
+  if (hipMalloc(&snap, snap_doubles * sizeof(double)) != hipSuccess)
+    return fail(ATHMI_ENOMEM, "athmi_autotune: no room for the state snapshot");
+  auto copy_state = [&](bool save) -> int {
+    size_t o = 0;
+    for (int k = 0; k < g->n; k++) {
+      const Inst& I = g->hinst[k];
+      double* live = I.state[hc[k].cur];
+      const size_t nb = NSTATE * I.ps * sizeof(double);
+      HIP_TRY(hipMemcpyAsync(save ? snap + o : live, save ? live : snap + o, nb,
+                             hipMemcpyDeviceToDevice, g->stream));
+      o += NSTATE * I.ps;
+    }
+    HIP_TRY(hipMemcpyAsync(g->dclk, hc.data(), sizeof(Clock) * g->n, hipMemcpyHostToDevice,
+                           g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return ATHMI_OK;
+  };
+  const LaunchShape old = g->bl.shape;
+  const bool old_graph = g->use_graph;
+  g->use_graph = true;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = copy_state(true);
+  double best = HUGE_VAL;
+  LaunchShape best_shape = old;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(ATHMI_EHIP, "athmi_autotune: event creation failed");
+  for (int a = 0; !rc && a < (int)(sizeof(kSeg) / sizeof(int)); a++) {
+    for (int b = 0; !rc && b < (int)(sizeof(kPersist) / sizeof(int)); b++) {
+      g->bl.shape.x2_seg = kSeg[a];
+      g->bl.shape.persist = kPersist[b];
+      if ((rc = g->refresh_tiles())) break;
+      if ((rc = athmi_step(g))) break;  // capture + first replay, untimed
+      if ((rc = copy_state(false))) break;
+      float ms = 0.f;
+      if (hipEventRecord(e0, g->stream) != hipSuccess) { rc = fail(ATHMI_EHIP, "event"); break; }
+      for (int s = 0; !rc && s < steps; s++) rc = athmi_step(g);
+      if (rc) break;
+      if (hipEventRecord(e1, g->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(ATHMI_EHIP, "athmi_autotune: event timing failed");
+        break;
+      }
+      if (ms / steps < best) {
+        best = ms / steps;
+        best_shape = g->bl.shape;
+      }
+      rc = copy_state(false);
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (!rc) rc = copy_state(false);
+  (void)hipFree(snap);
+  g->use_graph = old_graph;
+  g->bl.shape = rc ? old : best_shape;
+  int rc2 = g->refresh_tiles();
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (x2_seg) *x2_seg = best_shape.x2_seg;
+  if (persist) *persist = best_shape.persist;
+  if (ms_per_step) *ms_per_step = best;
+  return ATHMI_OK;
+}
+
 int athmi_use_graph(athmi_grid* g, int on) {
   if (!g) return fail(ATHMI_EINVAL, "null handle");
   g->use_graph = on != 0;

@@ -101,6 +101,43 @@ print(json.dumps(dict(steps=n, seconds=dt)))
                       f"({r['seconds']:.1f} s; host {cpu}, nproc {os.cpu_count()})"}
 
 
+def batched_leg(pkg, args, plain_rate):
+    """The batched uneven ensemble (seed 0): every instance runs its own CFL
+    dt for the timed steps; aggregate cell-updates/s over all instances."""
+    import importlib
+    ens = importlib.import_module(pkg.__name__ + ".ensemble")
+    params = ens.ensemble_params(seed=0, n=args.batch)
+    ncell = ens.cells(params)
+    log(f"[bench] batched ensemble: {args.batch} instances, {ncell} cells per step")
+    gb = ens.build(params, nlim=args.batch_warmup + args.batch_steps, variant=args.variant)
+    shape = None
+    if not args.no_tune:
+        shape = gb.autotune(2)
+        log(f"[bench] batch shape x2_seg={shape[0]} persist={shape[1]} ({shape[2]:.3f} ms/step)")
+    for _ in range(args.batch_warmup):
+        gb.step()
+    gb.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        gb.step()
+    gb.sync()
+    sec = time.perf_counter() - t0
+    steps = [gb.clock(k)[2] for k in range(gb.n)]
+    gb.close()
+    if min(steps) != max(steps) or steps[0] != args.batch_warmup + args.batch_steps:
+        raise RuntimeError(f"ensemble instances ran {min(steps)}..{max(steps)} steps")
+    rate = ncell * args.batch_steps / sec
+    log(f"[bench] batched {1e3 * sec / args.batch_steps:.3f} ms/step, {rate / 1e9:.3f} G "
+        f"cell-updates/s, {rate / plain_rate:.3f} of plain")
+    return {"value": rate, "unit": "cell-updates/s", "instances": args.batch,
+            "cells_per_step": ncell, "steps": args.batch_steps,
+            "ms_per_step": 1e3 * sec / args.batch_steps, "ratio_to_plain": rate / plain_rate,
+            "workload": "seeded uneven blast ensemble (seed 0): nx, ny in {64..1024 step 32}, "
+                        "off-centre blasts, Roe even / HLLE odd, per-instance CFL dt, "
+                        "one HIP graph per step",
+            "shape": None if shape is None else {"x2_seg": shape[0], "persist": shape[1]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +147,11 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=256,
+                    help="instances of the batched uneven ensemble (0: skip that leg)")
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--batch-warmup", type=int, default=3)
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy kernel whose HBM "
                          "counters calibrate FETCH_SIZE/WRITE_SIZE under rocprofv3 --pmc")
@@ -133,6 +175,10 @@ def main():
     g, par, tlim, _ = pkg.setup_problem("blast", [f"grid/Nx1={n}", f"grid/Nx2={n}"],
                                         variant=args.variant)
     g.set_limits(1e30, -1)  # fixed number of steps: the loop never reaches tlim
+    shape = None
+    if not args.no_tune:
+        shape = g.autotune(3)
+        log(f"[bench] autotuned shape x2_seg={shape[0]} persist={shape[1]} ({shape[2]:.3f} ms/step)")
     for _ in range(args.warmup):
         g.step()
     g.sync()
@@ -168,13 +214,16 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"MHD blast {n}x{n} (athinput.blast scaled), Roe + PLM, "
                                "CTU+CT, periodic, CFL 0.4; one step = integrate + new_dt + "
-                               "set_bvals", "nx1": n, "nx2": n, "variant": args.variant},
+                               "set_bvals", "nx1": n, "nx2": n, "variant": args.variant,
+                   "shape": None if shape is None else {"x2_seg": shape[0], "persist": shape[1]}},
         "roofline": roofline,
         "step_roofline": {"bound": "hbm", "alg_bytes_per_cell_update": ALG_BYTES_PER_CELL,
                           "achieved": step_bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_bw / HBM_PEAK_GBS},
     }
     g.close()
+    if args.batch > 0:
+        out["batched"] = batched_leg(pkg, args, value)
     if args.calibrate_copy:
         L.athmi_copy_calibration.argtypes = [ctypes.c_size_t, ctypes.c_int]
         pkg._check(L.athmi_copy_calibration(1 << 30, 3), "copy calibration")

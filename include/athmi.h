@@ -105,6 +105,12 @@ int athmi_sync(athmi_grid *g);
  * workgroups per CU (0 = one tile per workgroup). Picked by the autotuner. */
 int athmi_set_shape(athmi_grid *g, int x2_seg, int persist);
 int athmi_use_graph(athmi_grid *g, int on);
+/* Autotuner: time every launch shape (x2 marching depth 16..128 x persistent
+ * workgroups per CU 0..8) over `steps` main-loop steps from the current state
+ * through the step graph, restore the state and clocks, and keep the fastest
+ * shape (returned with its ms per step). No reference counterpart (the
+ * reference launches fixed 32-thread blocks, integrate_2d_cuda.cu:93-193). */
+int athmi_autotune(athmi_grid *g, int steps, int *x2_seg, int *persist, double *ms_per_step);
 /* Time nsteps main-loop steps kernel by kernel (HIP events on the handle's
  * stream between consecutive launches, no graph); avg_ms[k] receives the mean
  * duration of kernel k of the step (nslots >= 9), named by

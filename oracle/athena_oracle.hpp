@@ -1239,7 +1239,8 @@ void set_bvals(Grid<T>& G) {
 // ---------------------------------------------------------------------------
 // Problem generators (host libm, as the reference). blast.c:25-72
 template <class T>
-void pgen_blast(Grid<T>& G, double rin, double pa, double prat, double b0, double angle_deg) {
+void pgen_blast(Grid<T>& G, double rin, double pa, double prat, double b0, double angle_deg,
+                double cx = 0.0, double cy = 0.0) {
   using std::cos;
   using std::sin;
   using std::sqrt;
@@ -1263,7 +1264,10 @@ void pgen_blast(Grid<T>& G, double rin, double pa, double prat, double b0, doubl
       if (j == G.je && G.je > G.js) G.B2i[G.id(j + 1, i)] = T(bya);
       double x1, x2;
       cc_pos(G, i, j, &x1, &x2);
-      double rad = sqrt(x1 * x1 + x2 * x2);
+      // blast.c:64 centres the blast at the origin; (cx, cy) offsets it (cx = cy = 0
+      // reproduces the reference: x - 0 == x)
+      const double rx = x1 - cx, ry = x2 - cy;
+      double rad = sqrt(rx * rx + ry * ry);
       double pressure = pa;
       if (rad < rin) pressure = prat * pa;
       G.E[k] = T(pressure / G.eos.g1 + 0.5 * (bxa * bxa + bya * bya + bza * bza) +

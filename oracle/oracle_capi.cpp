@@ -47,6 +47,11 @@ void ora_pgen_blast(void* p, double radius, double pamb, double prat, double b0,
   pgen_blast(static_cast<Handle*>(p)->G, radius, pamb, prat, b0, angle);
 }
 
+void ora_pgen_blast_at(void* p, double radius, double pamb, double prat, double b0, double angle,
+                       double cx, double cy) {
+  pgen_blast(static_cast<Handle*>(p)->G, radius, pamb, prat, b0, angle, cx, cy);
+}
+
 void ora_pgen_field_loop(void* p, double rad, double amp, double vflow, int iprob) {
   pgen_field_loop(static_cast<Handle*>(p)->G, rad, amp, vflow, iprob);
 }

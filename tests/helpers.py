@@ -59,6 +59,7 @@ def oracle_lib():
         L.ora_put.argtypes = [_P, _P, _P, _P, _P]
         L.ora_set_time.argtypes = [_P, _D, _D, _I]
         L.ora_pgen_blast.argtypes = [_P, _D, _D, _D, _D, _D]
+        L.ora_pgen_blast_at.argtypes = [_P, _D, _D, _D, _D, _D, _D, _D]
         L.ora_pgen_field_loop.argtypes = [_P, _D, _D, _D, _I]
         L.ora_pgen_cpaw2d.argtypes = [_P, _D, _D, _D, _D, _D]
         L.ora_pgen_cpaw2d.restype = _D
