@@ -1,0 +1,132 @@
+// oracle_tlm.cpp -- TEST INFRASTRUCTURE ONLY: the oracle instantiated on dual
+// numbers, i.e. the forward-mode tangent-linear model (TLM) of the reference
+// routines it restates (esystem_prim.c, esystem_roe.c, flux_roe.c,
+// flux_hlle.c, convert_var.c, lr_states_plm.c, __integrate_2d__.c,
+// set_bvals.c). The adjoint kernels are checked against it by dot products:
+// <ybar, J xdot> from here equals <J^T ybar, xdot> from the adjoint.
+#include <cstring>
+#include <vector>
+
+#include "dual.hpp"
+#include "athena_oracle.hpp"
+
+using namespace ora;
+
+namespace {
+template <int N>
+void pack(Dual* out, const double* v, const double* t) {
+  for (int k = 0; k < N; k++) out[k] = Dual(v[k], t[k]);
+}
+template <int N>
+void unpack(const Dual* in, double* v, double* t) {
+  for (int k = 0; k < N; k++) {
+    v[k] = in[k].v;
+    t[k] = in[k].t;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+void ora_tlm_cons_to_prim(double gamma, const double* u, const double* du, double bx, double dbx,
+                          double* w, double* dw) {
+  Cons<Dual> U;
+  pack<7>(as_arr(U), u, du);
+  Prim<Dual> W = cons_to_prim(U, Dual(bx, dbx), Eos(gamma));
+  unpack<7>(as_arr(W), w, dw);
+}
+
+void ora_tlm_prim_to_cons(double gamma, const double* w, const double* dw, double bx, double dbx,
+                          double* u, double* du) {
+  Prim<Dual> W;
+  pack<7>(as_arr(W), w, dw);
+  Cons<Dual> U = prim_to_cons(W, Dual(bx, dbx), Eos(gamma));
+  unpack<7>(as_arr(U), u, du);
+}
+
+void ora_tlm_esys_prim(double gamma, const double* in, const double* din, double* ev, double* dev,
+                       double* R, double* dR, double* L, double* dL) {
+  Dual x[6], e[7], r[7][7], l[7][7];
+  pack<6>(x, in, din);
+  esys_prim<Dual>(x[0], x[1], x[2], x[3], x[4], x[5], Eos(gamma), e, r, l);
+  unpack<7>(e, ev, dev);
+  unpack<49>(&r[0][0], R, dR);
+  unpack<49>(&l[0][0], L, dL);
+}
+
+void ora_tlm_esys_roe(double gamma, const double* in, const double* din, double* ev, double* dev,
+                      double* R, double* dR, double* L, double* dL) {
+  Dual x[10], e[7], r[7][7], l[7][7];
+  pack<10>(x, in, din);
+  esys_roe<Dual>(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8], x[9], Eos(gamma), e, r,
+                 l);
+  unpack<7>(e, ev, dev);
+  unpack<49>(&r[0][0], R, dR);
+  unpack<49>(&l[0][0], L, dL);
+}
+
+// kind 0: flux_roe (returns its HLLE-fallback flag), 1: flux_hlle
+int ora_tlm_flux(double gamma, int kind, const double* ul, const double* dul, const double* ur,
+                 const double* dur, const double* wl, const double* dwl, const double* wr,
+                 const double* dwr, double bx, double dbx, double* f, double* df) {
+  Cons<Dual> Ul, Ur;
+  Prim<Dual> Wl, Wr;
+  pack<7>(as_arr(Ul), ul, dul);
+  pack<7>(as_arr(Ur), ur, dur);
+  pack<7>(as_arr(Wl), wl, dwl);
+  pack<7>(as_arr(Wr), wr, dwr);
+  int used = 0;
+  Cons<Dual> F = kind == 1 ? flux_hlle(Ul, Ur, Wl, Wr, Dual(bx, dbx), Eos(gamma))
+                           : flux_roe(Ul, Ur, Wl, Wr, Dual(bx, dbx), Eos(gamma), &used);
+  unpack<7>(as_arr(F), f, df);
+  return used;
+}
+
+void ora_tlm_lr_states(double gamma, int n, const double* W, const double* dW, const double* Bxc,
+                       const double* dBxc, double dtodx, int il, int iu, double* Wl, double* dWl,
+                       double* Wr, double* dWr) {
+  std::vector<Prim<Dual>> w(n), wl(n), wr(n);
+  std::vector<Dual> b(n);
+  for (int i = 0; i < n; i++) {
+    pack<7>(as_arr(w[i]), W + 7 * i, dW + 7 * i);
+    b[i] = Dual(Bxc[i], dBxc[i]);
+  }
+  lr_states(w.data(), b.data(), 0.0, dtodx, il, iu, wl.data(), wr.data(), Eos(gamma));
+  for (int i = 0; i < n; i++) {
+    unpack<7>(as_arr(wl[i]), Wl + 7 * i, dWl + 7 * i);
+    unpack<7>(as_arr(wr[i]), Wr + 7 * i, dWr + 7 * i);
+  }
+}
+
+// The rollout the adjoint differentiates: set_bvals, then nsteps times
+// (integrate_2d with the fixed dt, set_bvals). state/dstate: 11 planes
+// [Nx2+8][Nx1+8] in Grid order (d, M1, M2, M3, E, B1c, B2c, B3c, B1i, B2i, B3i).
+void ora_tlm_rollout(int nx1, int nx2, double x1min, double x1max, double x2min, double x2max,
+                     double gamma, const int* bc, int flux, double dt, int nsteps,
+                     const double* state, const double* dstate, double* out, double* dout) {
+  Grid<Dual> G;
+  init_grid(G, nx1, nx2, x1min, x1max, x2min, x2max, gamma, 0.4);
+  for (int k = 0; k < 4; k++) G.bc[k] = bc[k];
+  G.flux = flux;
+  const size_t np = (size_t)G.sx * G.sy;
+  for (int q = 0; q < 11; q++) {
+    std::vector<Dual>& p = *G.plane(q);
+    for (size_t k = 0; k < np; k++) p[k] = Dual(state[q * np + k], dstate[q * np + k]);
+  }
+  Scratch<Dual> S;
+  set_bvals(G);
+  G.dt = dt;
+  for (int s = 0; s < nsteps; s++) {
+    integrate_2d(G, S);
+    set_bvals(G);
+  }
+  for (int q = 0; q < 11; q++) {
+    const std::vector<Dual>& p = *G.plane(q);
+    for (size_t k = 0; k < np; k++) {
+      out[q * np + k] = p[k].v;
+      dout[q * np + k] = p[k].t;
+    }
+  }
+}
+
+}  // extern "C"

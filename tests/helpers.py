@@ -59,6 +59,14 @@ def oracle_lib():
         L.ora_put.argtypes = [_P, _P, _P, _P, _P]
         L.ora_set_time.argtypes = [_P, _D, _D, _I]
         L.ora_pgen_blast.argtypes = [_P, _D, _D, _D, _D, _D]
+        for name in ("ora_tlm_cons_to_prim", "ora_tlm_prim_to_cons"):
+            getattr(L, name).argtypes = [_D, _P, _P, _D, _D, _P, _P]
+        L.ora_tlm_esys_prim.argtypes = [_D] + [_P] * 8
+        L.ora_tlm_esys_roe.argtypes = [_D] + [_P] * 8
+        L.ora_tlm_flux.argtypes = [_D, _I] + [_P] * 8 + [_D, _D, _P, _P]
+        L.ora_tlm_flux.restype = _I
+        L.ora_tlm_lr_states.argtypes = [_D, _I, _P, _P, _P, _P, _D, _I, _I, _P, _P, _P, _P]
+        L.ora_tlm_rollout.argtypes = [_I, _I, _D, _D, _D, _D, _D, _P, _I, _D, _I, _P, _P, _P, _P]
         L.ora_pgen_blast_at.argtypes = [_P, _D, _D, _D, _D, _D, _D, _D]
         L.ora_pgen_field_loop.argtypes = [_P, _D, _D, _D, _I]
         L.ora_pgen_cpaw2d.argtypes = [_P, _D, _D, _D, _D, _D]
