@@ -1,0 +1,202 @@
+// adjoint_capi.cpp -- host runtime of the differentiation form (include/
+// athmi_adjoint.h): a fixed-dt rollout of N CTU+CT steps of one instance
+// (set_bvals, then N times integrate_2d + set_bvals) with a checkpoint every
+// K steps, and its reverse: per segment of K steps the states are recomputed
+// from the checkpoint, then each step is reversed (its forward scratch is
+// rebuilt from the stored input state by the forward kernels, and the adjoint
+// kernels run the stages backwards).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "adjoint_kernels.hpp"
+#include "athmi_adjoint.h"
+#include "runtime.hpp"
+
+namespace {
+
+// forward scratch of one step: predictor, predictor corner emf, corrector fluxes
+const int kSeqForwardScratch[] = {SK_BEGIN_ONCE, SK_PRED_X1, SK_PRED_X2, SK_CORNER_P, SK_CORR};
+const int kSeqReverse[] = {AK_FOLD_OUT,  AK_ZERO,       AK_UPDATE,   AK_CORNER_C, AK_RIEMANN_C,
+                           AK_HALF,      AK_TRANSVERSE, AK_CORNER_P, AK_EMF_CC,   AK_PRED_X1,
+                           AK_PLM_X1,    AK_PRED_X2,    AK_PLM_X2};
+
+}  // namespace
+
+struct athmi_adjoint {
+  athmi_grid* g = nullptr;
+  double dt = 0.0;
+  int nsteps = 0, every = 0, nck = 0;
+  int cur = 0;              // host copy of the clock's current buffer
+  bool have_forward = false;
+  double* mem = nullptr;    // checkpoints | segment states | adjoint state x2 | adjoint planes
+  double* ck = nullptr;
+  double* seg = nullptr;
+  double* adj[2] = {nullptr, nullptr};
+  double* ab = nullptr;
+  size_t state_doubles = 0;  // NSTATE * ps
+};
+
+static int adj_issue(athmi_adjoint* h, const int* seq, size_t n) {
+  return issue_seq(h->g, seq, n);
+}
+
+// one step of the rollout: integrate_2d with the clock's dt, then set_bvals
+static int rollout_step(athmi_adjoint* h) {
+  int rc = adj_issue(h, kSeqIntegrate, sizeof(kSeqIntegrate) / sizeof(int));
+  if (rc) return rc;
+  h->cur ^= 1;  // k_end swaps the buffers
+  HIP_TRY(h->g->launch(&h->g->bl, SK_GHOST_CUR, h->g->stream));
+  return ATHMI_OK;
+}
+
+static double* live_state(athmi_adjoint* h) { return h->g->hinst[0].state[h->cur]; }
+
+static int copy_state(athmi_adjoint* h, double* dst, const double* src) {
+  HIP_TRY(hipMemcpyAsync(dst, src, h->state_doubles * sizeof(double), hipMemcpyDeviceToDevice,
+                         h->g->stream));
+  return ATHMI_OK;
+}
+
+// dense [NSTATE][Nx2+8][Nx1+8] <-> pitched planes
+static int dense_to_planes(athmi_adjoint* h, double* planes, const double* dense) {
+  const Inst& I = h->g->hinst[0];
+  for (int q = 0; q < NSTATE; q++)
+    HIP_TRY(hipMemcpy2DAsync(planes + q * I.ps, I.pitch * sizeof(double),
+                             dense + (size_t)q * I.sx * I.sy, I.sx * sizeof(double),
+                             I.sx * sizeof(double), I.sy, hipMemcpyDeviceToDevice, h->g->stream));
+  return ATHMI_OK;
+}
+static int planes_to_dense(athmi_adjoint* h, double* dense, const double* planes) {
+  const Inst& I = h->g->hinst[0];
+  for (int q = 0; q < NSTATE; q++)
+    HIP_TRY(hipMemcpy2DAsync(dense + (size_t)q * I.sx * I.sy, I.sx * sizeof(double),
+                             planes + q * I.ps, I.pitch * sizeof(double), I.sx * sizeof(double),
+                             I.sy, hipMemcpyDeviceToDevice, h->g->stream));
+  return ATHMI_OK;
+}
+
+extern "C" {
+
+int athmi_adjoint_create(const athmi_desc* desc, double dt, int nsteps, int every,
+                         athmi_adjoint** out) {
+  if (!desc || !out || nsteps <= 0 || every <= 0 || !(dt > 0.0))
+    return fail(ATHMI_EINVAL, "athmi_adjoint_create: bad arguments");
+  if (desc->variant != ATHMI_EXACT)
+    return fail(ATHMI_EINVAL,
+                "athmi_adjoint_create: the adjoint linearises at the exact forward's branches; "
+                "use ATHMI_EXACT");
+  if (desc->nx1 < 4 || desc->nx2 < 4)
+    return fail(ATHMI_EINVAL, "athmi_adjoint_create: Nx1, Nx2 >= 4 (ghost sources interior)");
+  athmi_adjoint* h = new athmi_adjoint();
+  int rc = athmi_create(desc, 1, &h->g);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  h->dt = dt;
+  h->nsteps = nsteps;
+  h->every = std::min(every, nsteps);
+  h->nck = (nsteps + h->every - 1) / h->every;
+  const Inst& I = h->g->hinst[0];
+  h->state_doubles = (size_t)NSTATE * I.ps;
+  const size_t total = h->state_doubles * (h->nck + h->every + 2) + (size_t)NADJ * I.ps;
+  if (hipMalloc(&h->mem, total * sizeof(double)) != hipSuccess) {
+    athmi_destroy(h->g);
+    delete h;
+    return fail(ATHMI_ENOMEM, "athmi_adjoint_create: no room for checkpoints");
+  }
+  h->ck = h->mem;
+  h->seg = h->ck + h->state_doubles * h->nck;
+  h->adj[0] = h->seg + h->state_doubles * h->every;
+  h->adj[1] = h->adj[0] + h->state_doubles;
+  h->ab = h->adj[1] + h->state_doubles;
+  Clock c;
+  if ((rc = h->g->read_clock(0, c)) || (rc = athmi_set_clock(h->g, 0, dt, 0.0, 0))) {
+    athmi_adjoint_destroy(h);
+    return rc;
+  }
+  h->cur = c.cur;
+  *out = h;
+  return ATHMI_OK;
+}
+
+int athmi_adjoint_destroy(athmi_adjoint* h) {
+  if (!h) return ATHMI_OK;
+  if (h->g && h->g->stream) (void)hipStreamSynchronize(h->g->stream);
+  if (h->mem) (void)hipFree(h->mem);
+  if (h->g) athmi_destroy(h->g);
+  delete h;
+  return ATHMI_OK;
+}
+
+int athmi_adjoint_set_stream(athmi_adjoint* h, void* hip_stream) {
+  if (!h) return fail(ATHMI_EINVAL, "null handle");
+  return athmi_set_stream(h->g, hip_stream);
+}
+
+int athmi_adjoint_forward(athmi_adjoint* h, const double* state0, double* stateN) {
+  if (!h || !state0 || !stateN) return fail(ATHMI_EINVAL, "athmi_adjoint_forward: null argument");
+  int rc;
+  if ((rc = dense_to_planes(h, live_state(h), state0))) return rc;
+  HIP_TRY(h->g->launch(&h->g->bl, SK_GHOST_CUR, h->g->stream));
+  for (int n = 0; n < h->nsteps; n++) {
+    if (n % h->every == 0 &&
+        (rc = copy_state(h, h->ck + (size_t)(n / h->every) * h->state_doubles, live_state(h))))
+      return rc;
+    if ((rc = rollout_step(h))) return rc;
+  }
+  if ((rc = planes_to_dense(h, stateN, live_state(h)))) return rc;
+  HIP_TRY(hipStreamSynchronize(h->g->stream));
+  h->have_forward = true;
+  return ATHMI_OK;
+}
+
+int athmi_adjoint_backward(athmi_adjoint* h, const double* gradN, double* grad0) {
+  if (!h || !gradN || !grad0) return fail(ATHMI_EINVAL, "athmi_adjoint_backward: null argument");
+  if (!h->have_forward)
+    return fail(ATHMI_ESTATE, "athmi_adjoint_backward: run athmi_adjoint_forward first");
+  int rc;
+  double* aout = h->adj[0];
+  double* ain = h->adj[1];
+  if ((rc = dense_to_planes(h, aout, gradN))) return rc;
+  AdjArgs a;
+  a.g = h->g->hinst[0];
+  a.sc = a.g.scratch;
+  a.ab = h->ab;
+  a.dt = h->dt;
+  a.x2_seg = 32;
+  for (int s = h->nck - 1; s >= 0; s--) {
+    const int n0 = s * h->every, n1 = std::min(n0 + h->every, h->nsteps);
+    // states n0 .. n1-1 of this segment from its checkpoint
+    const double* ck = h->ck + (size_t)s * h->state_doubles;
+    if ((rc = copy_state(h, h->seg, ck)) || (rc = copy_state(h, live_state(h), ck))) return rc;
+    for (int m = n0 + 1; m < n1; m++) {
+      if ((rc = rollout_step(h))) return rc;
+      if ((rc = copy_state(h, h->seg + (size_t)(m - n0) * h->state_doubles, live_state(h))))
+        return rc;
+    }
+    for (int m = n1 - 1; m >= n0; m--) {
+      double* st = live_state(h);
+      if ((rc = copy_state(h, st, h->seg + (size_t)(m - n0) * h->state_doubles))) return rc;
+      if ((rc = adj_issue(h, kSeqForwardScratch, sizeof(kSeqForwardScratch) / sizeof(int))))
+        return rc;
+      a.st = st;
+      a.ain = ain;
+      a.aout = aout;
+      for (int k : kSeqReverse) HIP_TRY(athmi_adjoint_launch(&a, k, h->g->stream));
+      std::swap(ain, aout);
+    }
+  }
+  // the set_bvals that precedes the first step
+  a.aout = aout;
+  HIP_TRY(athmi_adjoint_launch(&a, AK_FOLD_OUT, h->g->stream));
+  if ((rc = planes_to_dense(h, grad0, aout))) return rc;
+  HIP_TRY(hipStreamSynchronize(h->g->stream));
+  return ATHMI_OK;
+}
+
+}  // extern "C"

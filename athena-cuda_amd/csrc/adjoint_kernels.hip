@@ -1,0 +1,744 @@
+// adjoint_kernels.hip -- reverse of one CTU+CT step (__integrate_2d__.c:100-537)
+// followed by set_bvals (set_bvals.c:114-1031), for one instance.
+//
+// Given the adjoint of the step's output state (aout) and the forward scratch
+// of the step recomputed from its input state S^n (predictor face states and
+// fluxes, predictor corner emf, corrector fluxes), the kernels below run the
+// step's stages in reverse and accumulate the adjoint of S^n into ain:
+//   fold        set_bvals: each ghost's adjoint is added to the cell it copies
+//   update      Steps 9, 11a, 11b, 13 -> corrector fluxes, corner emf, identity
+//   corner<C>   Step 9 corner emf -> corrector fluxes' By/Bz, emf3 at t^{n+1/2}
+//   riemann_c   Steps 8b/8c: corrector Riemann fluxes -> corrected face states
+//   half        Step 7: emf3 at t^{n+1/2} -> half-step face fields, predictor fluxes
+//   transverse  Steps 5, 6 and the half-step face fields of Step 4
+//   corner<P>   Step 4 corner emf -> predictor fluxes' By/Bz, emf3 at t^n
+//   emf_cc      Step 3
+//   pred_x1/x2  Steps 1c-1e / 2c-2e: sources, prim->cons, predictor fluxes
+//   plm_x1/x2   Steps 1a-1b / 2a-2b: cons->prim and the PLM reconstruction
+// Nonlinear stages recompute their forward values with the reference's
+// evaluation order (this file is built with -ffp-contract=off, like the exact
+// forward kernels), so every branch is the one the forward pass took. Linear
+// stages scatter with FP64 atomic adds; a value reached through several
+// stencil paths is summed in an unspecified order (roundoff-level).
+#include <hip/hip_runtime.h>
+
+#include "adjoint_device.hpp"
+#include "adjoint_kernels.hpp"
+#include "ghost_map.hpp"
+
+namespace athmi {
+namespace adjk {
+
+using namespace adj;
+
+__device__ __forceinline__ void ADD(double* plane, uint32_t k, double v) {
+  unsafeAtomicAdd(plane + k, v);
+}
+__device__ __forceinline__ Cons7 ld7(const double* base, size_t ps, uint32_t k) {
+  return Cons7{base[k], base[ps + k], base[2 * ps + k], base[3 * ps + k], base[4 * ps + k],
+               base[5 * ps + k], base[6 * ps + k]};
+}
+__device__ __forceinline__ void st7(double* base, size_t ps, uint32_t k, const Cons7& u) {
+  base[k] = u.d; base[ps + k] = u.mx; base[2 * ps + k] = u.my; base[3 * ps + k] = u.mz;
+  base[4 * ps + k] = u.e; base[5 * ps + k] = u.by; base[6 * ps + k] = u.bz;
+}
+__device__ __forceinline__ void st7(double* base, size_t ps, uint32_t k, const Prim7& w) {
+  base[k] = w.d; base[ps + k] = w.vx; base[2 * ps + k] = w.vy; base[3 * ps + k] = w.vz;
+  base[4 * ps + k] = w.p; base[5 * ps + k] = w.by; base[6 * ps + k] = w.bz;
+}
+__device__ __forceinline__ Prim7 ld7p(const double* base, size_t ps, uint32_t k) {
+  return Prim7{base[k], base[ps + k], base[2 * ps + k], base[3 * ps + k], base[4 * ps + k],
+               base[5 * ps + k], base[6 * ps + k]};
+}
+
+// (j, i) of this thread in the range [j0, j1] x [i0, i1] tiled kTileX x kTileY
+__device__ __forceinline__ bool range_ij(int j0, int j1, int i0, int i1, int& j, int& i) {
+  i = i0 + (int)blockIdx.x * kTileX + (int)threadIdx.x;
+  j = j0 + (int)blockIdx.y * kTileY + (int)threadIdx.y;
+  return i <= i1 && j <= j1;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kTileX* kTileY) ak_fold(AdjArgs a, int which) {
+  const Inst& g = a.g;
+  const int i = (int)blockIdx.x * kTileX + (int)threadIdx.x;
+  const int j = (int)blockIdx.y * kTileY + (int)threadIdx.y;
+  if (i >= g.sx || j >= g.sy) return;
+  double* A = which ? a.ain : a.aout;
+  const size_t ps = g.ps;
+  const uint32_t k = (uint32_t)(j * g.pitch + i);
+  for (int f = 0; f < NSTATE; f++) {
+    int sj, si;
+    bool neg;
+    if (ghost_src(g, f, j, i, sj, si, neg)) {
+      const double v = A[f * ps + k];
+      if (v != 0.0) ADD(A + f * ps, (uint32_t)(sj * g.pitch + si), neg ? -v : v);
+      A[f * ps + k] = 0.0;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) ak_zero(AdjArgs a) {
+  const size_t n_in = (size_t)NSTATE * a.g.ps, n_ab = (size_t)NADJ * a.g.ps;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_in + n_ab;
+       e += (size_t)gridDim.x * blockDim.x) {
+    if (e < n_in) a.ain[e] = 0.0;
+    else a.ab[e - n_in] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Steps 9, 11a, 11b, 13 (:473-534) over [js,je+1] x [is,ie+1]
+__global__ void __launch_bounds__(kTileX* kTileY) ak_update(AdjArgs a) {
+  const Inst& g = a.g;
+  int j, i;
+  if (!range_ij(g.js, g.je + 1, g.is, g.ie + 1, j, i)) return;
+  const size_t ps = g.ps;
+  const int pitch = g.pitch;
+  const double dtodx1 = a.dt / g.dx1, dtodx2 = a.dt / g.dx2;
+  const uint32_t k = (uint32_t)(j * pitch + i);
+  const double* ao = a.aout;
+  // adjoint of the updated value of cell field q at (jj, ii); 0 off the interior.
+  // B3i = B3c after the update (Step 13), so B3c carries both adjoints.
+  auto cb = [&](int q, int jj, int ii) -> double {
+    if (jj < g.js || jj > g.je || ii < g.is || ii > g.ie) return 0.0;
+    const uint32_t kk = (uint32_t)(jj * pitch + ii);
+    double v = ao[q * ps + kk];
+    if (q == F_B3C) v += ao[F_B3I * ps + kk];
+    return v;
+  };
+  if (j <= g.je && i <= g.ie) {
+    const int q_id[6] = {F_D, F_M1, F_M2, F_M3, F_E, F_B3C};
+    for (int n = 0; n < 6; n++) a.ain[q_id[n] * ps + k] += cb(q_id[n], j, i);
+  }
+  // corrector fluxes; the B2c update of 11a and the B1c update of 11b are
+  // overwritten by Step 13, so those components get no adjoint there
+  if (j <= g.je) {
+    const int q1[7] = {F_D, F_M1, F_M2, F_M3, F_E, -1, F_B3C};
+    double* F = a.ab + A_F1C * ps;
+    for (int n = 0; n < 7; n++)
+      F[n * ps + k] = (q1[n] < 0) ? 0.0 : dtodx1 * (cb(q1[n], j, i) - cb(q1[n], j, i - 1));
+  }
+  if (i <= g.ie) {
+    const int q2[7] = {F_D, F_M2, F_M3, F_M1, F_E, F_B3C, -1};
+    double* F = a.ab + A_F2C * ps;
+    for (int n = 0; n < 7; n++)
+      F[n * ps + k] = (q2[n] < 0) ? 0.0 : dtodx2 * (cb(q2[n], j, i) - cb(q2[n], j - 1, i));
+  }
+  // adjoint of the updated face fields including B1c, B2c = face averages
+  auto b1t = [&](int jj, int ii) -> double {
+    if (jj < g.js || jj > g.je || ii < g.is || ii > g.ie + 1) return 0.0;
+    return ao[F_B1I * ps + (uint32_t)(jj * pitch + ii)] +
+           0.5 * (cb(F_B1C, jj, ii) + cb(F_B1C, jj, ii - 1));
+  };
+  auto b2t = [&](int jj, int ii) -> double {
+    if (jj < g.js || jj > g.je + 1 || ii < g.is || ii > g.ie) return 0.0;
+    return ao[F_B2I * ps + (uint32_t)(jj * pitch + ii)] +
+           0.5 * (cb(F_B2C, jj, ii) + cb(F_B2C, jj - 1, ii));
+  };
+  if (j <= g.je) a.ain[F_B1I * ps + k] += b1t(j, i);
+  if (i <= g.ie) a.ain[F_B2I * ps + k] += b2t(j, i);
+  a.ab[A_EMFC * ps + k] = dtodx2 * (b1t(j, i) - b1t(j - 1, i)) + dtodx1 * (b2t(j, i - 1) - b2t(j, i));
+}
+
+// ---------------------------------------------------------------------------
+// integrate_emf3_corner (:632-702) reversed. PRED: Step 4 over
+// [js-1,je+2]x[is-1,ie+2]; otherwise Step 9 over [js,je+1]x[is,ie+1].
+__device__ __forceinline__ double upwind_weight(double mass_flux) {
+  return (mass_flux > 0.0) ? 1.0 : ((mass_flux < 0.0) ? 0.0 : 0.5);
+}
+
+template <bool PRED>
+__global__ void __launch_bounds__(kTileX* kTileY) ak_corner(AdjArgs a) {
+  const Inst& g = a.g;
+  int j, i;
+  if (PRED) {
+    if (!range_ij(g.js - 1, g.je + 2, g.is - 1, g.ie + 2, j, i)) return;
+  } else {
+    if (!range_ij(g.js, g.je + 1, g.is, g.ie + 1, j, i)) return;
+  }
+  const size_t ps = g.ps;
+  const int pitch = g.pitch;
+  const uint32_t k = (uint32_t)(j * pitch + i);
+  const double q = 0.25 * a.ab[(PRED ? A_EMFP : A_EMFC) * ps + k];
+  if (q == 0.0) return;
+  const double* F1 = a.sc + (PRED ? S_F1 : S_F1C) * ps;  // mass fluxes pick the upwind side
+  const double* F2 = a.sc + (PRED ? S_F2 : S_F2C) * ps;
+  double* F1b = a.ab + (PRED ? A_F1 : A_F1C) * ps;
+  double* F2b = a.ab + (PRED ? A_F2 : A_F2C) * ps;
+  double* eb = a.ab + (PRED ? A_ECC : A_ECH) * ps;
+  const uint32_t kjm = k - pitch, kim = k - 1, kjmim = k - pitch - 1;
+  double c_f1jm = 0, c_f1 = 0, c_f2im = 0, c_f2 = 0, c_ejmim = 0, c_ejm = 0, c_eim = 0, c_e00 = 0;
+  double w = upwind_weight(F1[kjm]);  // emf_l2
+  c_f1jm -= 1.0; c_f2im += w; c_ejmim -= w; c_f2 += 1.0 - w; c_ejm -= 1.0 - w;
+  w = upwind_weight(F1[k]);  // emf_r2
+  c_f1 -= 1.0; c_f2im += w; c_eim -= w; c_f2 += 1.0 - w; c_e00 -= 1.0 - w;
+  w = upwind_weight(F2[kim]);  // emf_l1
+  c_f2im += 1.0; c_f1jm -= w; c_ejmim -= w; c_f1 -= 1.0 - w; c_eim -= 1.0 - w;
+  w = upwind_weight(F2[k]);  // emf_r1
+  c_f2 += 1.0; c_f1jm -= w; c_ejm -= w; c_f1 -= 1.0 - w; c_e00 -= 1.0 - w;
+  ADD(F1b + 5 * ps, kjm, q * c_f1jm);
+  ADD(F1b + 5 * ps, k, q * c_f1);
+  ADD(F2b + 6 * ps, kim, q * c_f2im);
+  ADD(F2b + 6 * ps, k, q * c_f2);
+  ADD(eb, kjmim, q * c_ejmim);
+  ADD(eb, kjm, q * c_ejm);
+  ADD(eb, kim, q * c_eim);
+  ADD(eb, k, q * c_e00);
+}
+
+// ---------------------------------------------------------------------------
+// Forward values shared by the corrector stages: half-step face fields
+// (:266-275) and the corrected face states (Steps 5, 6), in the exact forward
+// order.
+struct CorrCtx {
+  const Inst& g;
+  const double* st;
+  const double* sc;
+  size_t ps;
+  int pitch;
+  double hdtodx1, hdtodx2;
+  __device__ double b1h(uint32_t kk) const {
+    const double* emf = sc + S_EMF * ps;
+    return st[F_B1I * ps + kk] - hdtodx2 * (emf[kk + pitch] - emf[kk]);
+  }
+  __device__ double b2h(uint32_t kk) const {
+    const double* emf = sc + S_EMF * ps;
+    return st[F_B2I * ps + kk] + hdtodx1 * (emf[kk + 1] - emf[kk]);
+  }
+  __device__ double s(int f, uint32_t kk) const { return st[f * ps + kk]; }
+  // x1 face k: Step 5a with x2 fluxes at (j+1,i-1),(j,i-1) / (j+1,i),(j,i) and 5b
+  __device__ void states_x1(uint32_t k, Cons7& L, Cons7& R) const {
+    const double* F2 = sc + S_F2 * ps;
+    L = ld7(sc + S_UL1 * ps, ps, k);
+    R = ld7(sc + S_UR1 * ps, ps, k);
+    const uint32_t a1 = k + pitch - 1, a0 = k - 1, c1 = k + pitch, c0 = k;
+    L.d = L.d - hdtodx2 * (F2[a1] - F2[a0]);
+    L.mx = L.mx - hdtodx2 * (F2[3 * ps + a1] - F2[3 * ps + a0]);
+    L.my = L.my - hdtodx2 * (F2[1 * ps + a1] - F2[1 * ps + a0]);
+    L.mz = L.mz - hdtodx2 * (F2[2 * ps + a1] - F2[2 * ps + a0]);
+    L.e = L.e - hdtodx2 * (F2[4 * ps + a1] - F2[4 * ps + a0]);
+    L.bz = L.bz - hdtodx2 * (F2[5 * ps + a1] - F2[5 * ps + a0]);
+    R.d = R.d - hdtodx2 * (F2[c1] - F2[c0]);
+    R.mx = R.mx - hdtodx2 * (F2[3 * ps + c1] - F2[3 * ps + c0]);
+    R.my = R.my - hdtodx2 * (F2[1 * ps + c1] - F2[1 * ps + c0]);
+    R.mz = R.mz - hdtodx2 * (F2[2 * ps + c1] - F2[2 * ps + c0]);
+    R.e = R.e - hdtodx2 * (F2[4 * ps + c1] - F2[4 * ps + c0]);
+    R.bz = R.bz - hdtodx2 * (F2[5 * ps + c1] - F2[5 * ps + c0]);
+    const double hdtodx1_ = hdtodx1;
+    double dbx = s(F_B1I, k) - s(F_B1I, k - 1);
+    double b1 = s(F_B1C, k - 1), b2 = s(F_B2C, k - 1), b3 = s(F_B3C, k - 1);
+    double v3 = s(F_M3, k - 1) / s(F_D, k - 1);
+    L.mx = L.mx + hdtodx1_ * b1 * dbx;
+    L.my = L.my + hdtodx1_ * b2 * dbx;
+    L.mz = L.mz + hdtodx1_ * b3 * dbx;
+    L.bz = L.bz + hdtodx1_ * v3 * dbx;
+    L.e = L.e + hdtodx1_ * b3 * v3 * dbx;
+    dbx = s(F_B1I, k + 1) - s(F_B1I, k);
+    b1 = s(F_B1C, k); b2 = s(F_B2C, k); b3 = s(F_B3C, k);
+    v3 = s(F_M3, k) / s(F_D, k);
+    R.mx = R.mx + hdtodx1_ * b1 * dbx;
+    R.my = R.my + hdtodx1_ * b2 * dbx;
+    R.mz = R.mz + hdtodx1_ * b3 * dbx;
+    R.bz = R.bz + hdtodx1_ * v3 * dbx;
+    R.e = R.e + hdtodx1_ * b3 * v3 * dbx;
+  }
+  // x2 face k: Step 6a with x1 fluxes at (j-1,i+1),(j-1,i) / (j,i+1),(j,i) and 6b
+  __device__ void states_x2(uint32_t k, Cons7& L, Cons7& R) const {
+    const double* F1 = sc + S_F1 * ps;
+    L = ld7(sc + S_UL2 * ps, ps, k);
+    R = ld7(sc + S_UR2 * ps, ps, k);
+    const uint32_t a1 = k - pitch + 1, a0 = k - pitch, c1 = k + 1, c0 = k;
+    L.d = L.d - hdtodx1 * (F1[a1] - F1[a0]);
+    L.mx = L.mx - hdtodx1 * (F1[2 * ps + a1] - F1[2 * ps + a0]);
+    L.my = L.my - hdtodx1 * (F1[3 * ps + a1] - F1[3 * ps + a0]);
+    L.mz = L.mz - hdtodx1 * (F1[1 * ps + a1] - F1[1 * ps + a0]);
+    L.e = L.e - hdtodx1 * (F1[4 * ps + a1] - F1[4 * ps + a0]);
+    L.by = L.by - hdtodx1 * (F1[6 * ps + a1] - F1[6 * ps + a0]);
+    R.d = R.d - hdtodx1 * (F1[c1] - F1[c0]);
+    R.mx = R.mx - hdtodx1 * (F1[2 * ps + c1] - F1[2 * ps + c0]);
+    R.my = R.my - hdtodx1 * (F1[3 * ps + c1] - F1[3 * ps + c0]);
+    R.mz = R.mz - hdtodx1 * (F1[1 * ps + c1] - F1[1 * ps + c0]);
+    R.e = R.e - hdtodx1 * (F1[4 * ps + c1] - F1[4 * ps + c0]);
+    R.by = R.by - hdtodx1 * (F1[6 * ps + c1] - F1[6 * ps + c0]);
+    const uint32_t km = k - pitch;
+    double dby = s(F_B2I, k) - s(F_B2I, km);
+    double b1 = s(F_B1C, km), b2 = s(F_B2C, km), b3 = s(F_B3C, km);
+    double v3 = s(F_M3, km) / s(F_D, km);
+    L.mz = L.mz + hdtodx2 * b1 * dby;
+    L.mx = L.mx + hdtodx2 * b2 * dby;
+    L.my = L.my + hdtodx2 * b3 * dby;
+    L.by = L.by + hdtodx2 * v3 * dby;
+    L.e = L.e + hdtodx2 * b3 * v3 * dby;
+    dby = s(F_B2I, k + pitch) - s(F_B2I, k);
+    b1 = s(F_B1C, k); b2 = s(F_B2C, k); b3 = s(F_B3C, k);
+    v3 = s(F_M3, k) / s(F_D, k);
+    R.mz = R.mz + hdtodx2 * b1 * dby;
+    R.mx = R.mx + hdtodx2 * b2 * dby;
+    R.my = R.my + hdtodx2 * b3 * dby;
+    R.by = R.by + hdtodx2 * v3 * dby;
+    R.e = R.e + hdtodx2 * b3 * v3 * dby;
+  }
+};
+
+__device__ __forceinline__ CorrCtx corr_ctx(const AdjArgs& a) {
+  const double dtodx1 = a.dt / a.g.dx1, dtodx2 = a.dt / a.g.dx2;
+  return CorrCtx{a.g, a.st, a.sc, a.g.ps, a.g.pitch, 0.5 * dtodx1, 0.5 * dtodx2};
+}
+
+// Steps 8b, 8c (:438-464) over [js-1,je+1] x [is-1,ie+1]
+__global__ void __launch_bounds__(kTileX* kTileY) ak_riemann_c(AdjArgs a) {
+  const Inst& g = a.g;
+  int j, i;
+  if (!range_ij(g.js - 1, g.je + 1, g.is - 1, g.ie + 1, j, i)) return;
+  const CorrCtx c = corr_ctx(a);
+  const size_t ps = g.ps;
+  const uint32_t k = (uint32_t)(j * g.pitch + i);
+  const EosC eos{g.gamma, g.g1, g.g2};
+  if (i >= g.is) {
+    Cons7 L, R;
+    c.states_x1(k, L, R);
+    const double bx = c.b1h(k);
+    const Prim7 wl = cons_to_prim(L, bx, eos), wr = cons_to_prim(R, bx, eos);
+    Cons7 Lb = zero_cons(), Rb = zero_cons();
+    Prim7 Wlb = zero_prim(), Wrb = zero_prim();
+    double bxb = 0.0;
+    riemann_b(g.flux, L, R, wl, wr, bx, eos, ld7(a.ab + A_F1C * ps, ps, k), Lb, Rb, Wlb, Wrb, bxb);
+    cons_to_prim_b(L, bx, eos, Wlb, Lb, bxb);
+    cons_to_prim_b(R, bx, eos, Wrb, Rb, bxb);
+    st7(a.ab + A_UL1 * ps, ps, k, Lb);
+    st7(a.ab + A_UR1 * ps, ps, k, Rb);
+    ADD(a.ab + A_B1H * ps, k, bxb);
+  }
+  if (j >= g.js) {
+    Cons7 L, R;
+    c.states_x2(k, L, R);
+    const double bx = c.b2h(k);
+    const Prim7 wl = cons_to_prim(L, bx, eos), wr = cons_to_prim(R, bx, eos);
+    Cons7 Lb = zero_cons(), Rb = zero_cons();
+    Prim7 Wlb = zero_prim(), Wrb = zero_prim();
+    double bxb = 0.0;
+    riemann_b(g.flux, L, R, wl, wr, bx, eos, ld7(a.ab + A_F2C * ps, ps, k), Lb, Rb, Wlb, Wrb, bxb);
+    cons_to_prim_b(L, bx, eos, Wlb, Lb, bxb);
+    cons_to_prim_b(R, bx, eos, Wrb, Rb, bxb);
+    st7(a.ab + A_UL2 * ps, ps, k, Lb);
+    st7(a.ab + A_UR2 * ps, ps, k, Rb);
+    ADD(a.ab + A_B2H * ps, k, bxb);
+  }
+}
+
+// Step 7 (:402-431) over [js-1,je+1] x [is-1,ie+1]
+__global__ void __launch_bounds__(kTileX* kTileY) ak_half(AdjArgs a) {
+  const Inst& g = a.g;
+  int j, i;
+  if (!range_ij(g.js - 1, g.je + 1, g.is - 1, g.ie + 1, j, i)) return;
+  const size_t ps = g.ps;
+  const int pitch = g.pitch;
+  const uint32_t k = (uint32_t)(j * pitch + i);
+  const double Eb = a.ab[A_ECH * ps + k];
+  if (Eb == 0.0) return;
+  const CorrCtx c = corr_ctx(a);
+  const double h1 = c.hdtodx1, h2 = c.hdtodx2;
+  const double* F1 = a.sc + S_F1 * ps;
+  const double* F2 = a.sc + S_F2 * ps;
+  const double dhalf = c.s(F_D, k) - h1 * (F1[k + 1] - F1[k]) - h2 * (F2[k + pitch] - F2[k]);
+  const double m1 = c.s(F_M1, k) - h1 * (F1[ps + k + 1] - F1[ps + k]) -
+                    h2 * (F2[3 * ps + k + pitch] - F2[3 * ps + k]);
+  const double m2 = c.s(F_M2, k) - h1 * (F1[2 * ps + k + 1] - F1[2 * ps + k]) -
+                    h2 * (F2[ps + k + pitch] - F2[ps + k]);
+  const double b1c = 0.5 * (c.b1h(k) + c.b1h(k + 1));
+  const double b2c = 0.5 * (c.b2h(k) + c.b2h(k + pitch));
+  const double emf = (b1c * m2 - b2c * m1) / dhalf;
+  const double b1cb = Eb * m2 / dhalf, m2b = Eb * b1c / dhalf;
+  const double b2cb = -Eb * m1 / dhalf, m1b = -Eb * b2c / dhalf;
+  const double dhb = -Eb * emf / dhalf;
+  double* B1hb = a.ab + A_B1H * ps;
+  double* B2hb = a.ab + A_B2H * ps;
+  ADD(B1hb, k, 0.5 * b1cb);
+  ADD(B1hb, k + 1, 0.5 * b1cb);
+  ADD(B2hb, k, 0.5 * b2cb);
+  ADD(B2hb, k + pitch, 0.5 * b2cb);
+  double* F1b = a.ab + A_F1 * ps;
+  double* F2b = a.ab + A_F2 * ps;
+  ADD(a.ain + F_D * ps, k, dhb);
+  ADD(F1b, k + 1, -h1 * dhb);
+  ADD(F1b, k, h1 * dhb);
+  ADD(F2b, k + pitch, -h2 * dhb);
+  ADD(F2b, k, h2 * dhb);
+  ADD(a.ain + F_M1 * ps, k, m1b);
+  ADD(F1b + ps, k + 1, -h1 * m1b);
+  ADD(F1b + ps, k, h1 * m1b);
+  ADD(F2b + 3 * ps, k + pitch, -h2 * m1b);
+  ADD(F2b + 3 * ps, k, h2 * m1b);
+  ADD(a.ain + F_M2 * ps, k, m2b);
+  ADD(F1b + 2 * ps, k + 1, -h1 * m2b);
+  ADD(F1b + 2 * ps, k, h1 * m2b);
+  ADD(F2b + ps, k + pitch, -h2 * m2b);
+  ADD(F2b + ps, k, h2 * m2b);
+}
+
+// Steps 5, 6 (:284-393) and the half-step face fields of Step 4 (:263-275)
+// over [js-1,je+2] x [is-1,ie+2]
+__global__ void __launch_bounds__(kTileX* kTileY) ak_transverse(AdjArgs a) {
+  const Inst& g = a.g;
+  int j, i;
+  if (!range_ij(g.js - 1, g.je + 2, g.is - 1, g.ie + 2, j, i)) return;
+  const size_t ps = g.ps;
+  const int pitch = g.pitch;
+  const uint32_t k = (uint32_t)(j * pitch + i);
+  const double h1 = 0.5 * a.dt / g.dx1, h2 = 0.5 * a.dt / g.dx2;
+  const double* st = a.st;
+  double* ain = a.ain;
+  // MHD source terms of one face state with the cell-centred field of cell kc
+  // and the face-field difference B[kp] - B[km]; perm: components of the
+  // state receiving (b1, b2, b3) and the transverse-field slot
+  auto src_b = [&](const Cons7& Xb, uint32_t kc, int bplane, uint32_t kp, uint32_t km,
+                   double h, double xb1, double xb2, double xb3, double xbt) {
+    const double db = st[bplane * ps + kp] - st[bplane * ps + km];
+    const double b1 = st[F_B1C * ps + kc], b2 = st[F_B2C * ps + kc], b3 = st[F_B3C * ps + kc];
+    const double d = st[F_D * ps + kc], v3 = st[F_M3 * ps + kc] / d;
+    const double dbb = h * (xb1 * b1 + xb2 * b2 + xb3 * b3 + xbt * v3 + Xb.e * b3 * v3);
+    ADD(ain + bplane * ps, kp, dbb);
+    ADD(ain + bplane * ps, km, -dbb);
+    ADD(ain + F_B1C * ps, kc, h * db * xb1);
+    ADD(ain + F_B2C * ps, kc, h * db * xb2);
+    ADD(ain + F_B3C * ps, kc, h * db * (xb3 + Xb.e * v3));
+    const double v3b = h * db * (xbt + Xb.e * b3);
+    ADD(ain + F_M3 * ps, kc, v3b / d);
+    ADD(ain + F_D * ps, kc, -v3b * v3 / d);
+  };
+  if (i >= g.is && i <= g.ie + 1 && j <= g.je + 1) {  // corrected x1-face states
+    double* F2b = a.ab + A_F2 * ps;
+    const Cons7 Lb = ld7(a.ab + A_UL1 * ps, ps, k), Rb = ld7(a.ab + A_UR1 * ps, ps, k);
+    auto flux_b = [&](const Cons7& Xb, uint32_t f1, uint32_t f0) {
+      // X.d,mx,my,mz,e,bz -= h2 * (F2 d, mz, mx, my, e, by)[f1] - [f0]
+      const int comp[6] = {0, 3, 1, 2, 4, 5};
+      const double v[6] = {Xb.d, Xb.mx, Xb.my, Xb.mz, Xb.e, Xb.bz};
+      for (int n = 0; n < 6; n++) {
+        ADD(F2b + comp[n] * ps, f1, -h2 * v[n]);
+        ADD(F2b + comp[n] * ps, f0, h2 * v[n]);
+      }
+    };
+    flux_b(Lb, k + pitch - 1, k - 1);
+    flux_b(Rb, k + pitch, k);
+    src_b(Lb, k - 1, F_B1I, k, k - 1, h1, Lb.mx, Lb.my, Lb.mz, Lb.bz);
+    src_b(Rb, k, F_B1I, k + 1, k, h1, Rb.mx, Rb.my, Rb.mz, Rb.bz);
+  }
+  if (j >= g.js && j <= g.je + 1 && i <= g.ie + 1) {  // corrected x2-face states
+    double* F1b = a.ab + A_F1 * ps;
+    const Cons7 Lb = ld7(a.ab + A_UL2 * ps, ps, k), Rb = ld7(a.ab + A_UR2 * ps, ps, k);
+    auto flux_b = [&](const Cons7& Xb, uint32_t f1, uint32_t f0) {
+      // X.d,mx,my,mz,e,by -= h1 * (F1 d, my, mz, mx, e, bz)[f1] - [f0]
+      const int comp[6] = {0, 2, 3, 1, 4, 6};
+      const double v[6] = {Xb.d, Xb.mx, Xb.my, Xb.mz, Xb.e, Xb.by};
+      for (int n = 0; n < 6; n++) {
+        ADD(F1b + comp[n] * ps, f1, -h1 * v[n]);
+        ADD(F1b + comp[n] * ps, f0, h1 * v[n]);
+      }
+    };
+    flux_b(Lb, k - pitch + 1, k - pitch);
+    flux_b(Rb, k + 1, k);
+    // x2 states are (d, M2, M3, M1, E, B3, B1): b1 enters M1 = mz, b2 enters
+    // M2 = mx, b3 enters M3 = my, v3 enters B3 = by
+    src_b(Lb, k - pitch, F_B2I, k, k - pitch, h2, Lb.mz, Lb.mx, Lb.my, Lb.by);
+    src_b(Rb, k, F_B2I, k + pitch, k, h2, Rb.mz, Rb.mx, Rb.my, Rb.by);
+  }
+  if (j <= g.je + 1) {  // B1h = B1i - h2 (emf[j+1] - emf[j]) on [js-1,je+1] x [is-1,ie+2]
+    const double hb = a.ab[A_B1H * ps + k];
+    if (hb != 0.0) {
+      ADD(ain + F_B1I * ps, k, hb);
+      ADD(a.ab + A_EMFP * ps, k + pitch, -h2 * hb);
+      ADD(a.ab + A_EMFP * ps, k, h2 * hb);
+    }
+  }
+  if (i <= g.ie + 1) {  // B2h = B2i + h1 (emf[i+1] - emf[i]) on [js-1,je+2] x [is-1,ie+1]
+    const double hb = a.ab[A_B2H * ps + k];
+    if (hb != 0.0) {
+      ADD(ain + F_B2I * ps, k, hb);
+      ADD(a.ab + A_EMFP * ps, k + 1, h1 * hb);
+      ADD(a.ab + A_EMFP * ps, k, -h1 * hb);
+    }
+  }
+}
+
+// Step 3 (:249-255) over [js-2,je+2] x [is-2,ie+2]
+__global__ void __launch_bounds__(kTileX* kTileY) ak_emf_cc(AdjArgs a) {
+  const Inst& g = a.g;
+  int j, i;
+  if (!range_ij(g.js - 2, g.je + 2, g.is - 2, g.ie + 2, j, i)) return;
+  const size_t ps = g.ps;
+  const uint32_t k = (uint32_t)(j * g.pitch + i);
+  const double Eb = a.ab[A_ECC * ps + k];
+  if (Eb == 0.0) return;
+  const double* st = a.st;
+  const double d = st[F_D * ps + k], m1 = st[F_M1 * ps + k], m2 = st[F_M2 * ps + k];
+  const double b1 = st[F_B1C * ps + k], b2 = st[F_B2C * ps + k];
+  const double e = (b1 * m2 - b2 * m1) / d;
+  ADD(a.ain + F_B1C * ps, k, Eb * m2 / d);
+  ADD(a.ain + F_M2 * ps, k, Eb * b1 / d);
+  ADD(a.ain + F_B2C * ps, k, -Eb * m1 / d);
+  ADD(a.ain + F_M1 * ps, k, -Eb * b2 / d);
+  ADD(a.ain + F_D * ps, k, -Eb * e / d);
+}
+
+// ---------------------------------------------------------------------------
+// Predictor faces: the face state pair (PLM outputs wl, wr + the 1/2 dt MHD
+// source on the transverse field + prim->cons) and its Riemann flux, reversed.
+// bplane: the normal face field; mplane: the momentum of the source's
+// velocity; src_slot: component of Prim7 receiving the source (by for x1, bz
+// for x2). Returns the adjoints of the PLM outputs in wlb, wrb.
+__device__ __forceinline__ void pred_face_b(const AdjArgs& a, uint32_t k, uint32_t stride,
+                                            Prim7 wl, Prim7 wr, double dx, int bplane,
+                                            int mplane, bool x1, const Cons7& Fb, Cons7 Ulb,
+                                            Cons7 Urb, Prim7& wlb, Prim7& wrb) {
+  const Inst& g = a.g;
+  const size_t ps = g.ps;
+  const double* st = a.st;
+  const EosC eos{g.gamma, g.g1, g.g2};
+  const double hdt = 0.5 * a.dt;
+  const double bm = st[bplane * ps + k - stride], b0 = st[bplane * ps + k];
+  const double bp = st[bplane * ps + k + stride];
+  const double dl = st[F_D * ps + k - stride], dr = st[F_D * ps + k];
+  const double vl = st[mplane * ps + k - stride] / dl, vr = st[mplane * ps + k] / dr;
+  double srcl = vl * (b0 - bm) / dx;
+  double srcr = vr * (bp - b0) / dx;
+  if (x1) {
+    wl.by = wl.by + hdt * srcl;
+    wr.by = wr.by + hdt * srcr;
+  } else {
+    wl.bz = wl.bz + hdt * srcl;
+    wr.bz = wr.bz + hdt * srcr;
+  }
+  const Cons7 ul = prim_to_cons(wl, b0, eos), ur = prim_to_cons(wr, b0, eos);
+  wlb = zero_prim();
+  wrb = zero_prim();
+  double b0b = 0.0;
+  riemann_b(g.flux, ul, ur, wl, wr, b0, eos, Fb, Ulb, Urb, wlb, wrb, b0b);
+  prim_to_cons_b(wl, b0, eos, Ulb, wlb, b0b);
+  prim_to_cons_b(wr, b0, eos, Urb, wrb, b0b);
+  const double slb = hdt * (x1 ? wlb.by : wlb.bz), srb = hdt * (x1 ? wrb.by : wrb.bz);
+  const double dblb = slb * vl / dx, vlb = slb * (b0 - bm) / dx;
+  const double dbrb = srb * vr / dx, vrb = srb * (bp - b0) / dx;
+  ADD(a.ain + bplane * ps, k - stride, -dblb);
+  ADD(a.ain + bplane * ps, k, b0b + dblb - dbrb);
+  ADD(a.ain + bplane * ps, k + stride, dbrb);
+  ADD(a.ain + mplane * ps, k - stride, vlb / dl);
+  ADD(a.ain + F_D * ps, k - stride, -vlb * vl / dl);
+  ADD(a.ain + mplane * ps, k, vrb / dr);
+  ADD(a.ain + F_D * ps, k, -vrb * vr / dr);
+}
+
+__device__ __forceinline__ Prim7 lds_get(double (*s)[kX1Block], int t) {
+  return Prim7{s[0][t], s[1][t], s[2][t], s[3][t], s[4][t], s[5][t], s[6][t]};
+}
+__device__ __forceinline__ void lds_set(double (*s)[kX1Block], int t, const Prim7& w) {
+  s[0][t] = w.d; s[1][t] = w.vx; s[2][t] = w.vy; s[3][t] = w.vz;
+  s[4][t] = w.p; s[5][t] = w.by; s[6][t] = w.bz;
+}
+
+// Steps 1c-1e reversed: one row segment of x1 faces per workgroup (the
+// forward k_pred_x1 layout: 256 cells -> 253 faces)
+__global__ void __launch_bounds__(kX1Block) ak_pred_x1(AdjArgs a) {
+  __shared__ double sw[7][kX1Block];
+  __shared__ double swl[7][kX1Block];
+  __shared__ double swr[7][kX1Block];
+  const Inst& g = a.g;
+  const int tid = threadIdx.x;
+  const int nseg = (g.nx1 + 4 + kX1Faces - 1) / kX1Faces;
+  const int row = (int)blockIdx.x / nseg, seg = (int)blockIdx.x - row * nseg;
+  const int j = g.js - 2 + row;
+  const int f0 = g.is - 1 + seg * kX1Faces;
+  const int cell = f0 - 2 + tid;
+  const double dtodx1 = a.dt / g.dx1;
+  const EosC eos{g.gamma, g.g1, g.g2};
+  const double* st = a.st;
+  const size_t ps = g.ps;
+  const uint32_t k = (uint32_t)(j * g.pitch + cell);
+  double bxc = 0.0;
+  if (cell >= g.is - 3 && cell <= g.ie + 3) {
+    const Cons7 u{st[F_D * ps + k], st[F_M1 * ps + k], st[F_M2 * ps + k], st[F_M3 * ps + k],
+                  st[F_E * ps + k], st[F_B2C * ps + k], st[F_B3C * ps + k]};
+    bxc = st[F_B1C * ps + k];
+    lds_set(sw, tid, cons_to_prim(u, bxc, eos));
+  }
+  __syncthreads();
+  if (tid >= 1 && tid <= kX1Block - 2 && cell >= g.is - 2 && cell <= g.ie + 2) {
+    Prim7 wl, wr;
+    plm_cell(lds_get(sw, tid - 1), lds_get(sw, tid), lds_get(sw, tid + 1), bxc, dtodx1, eos, wl,
+             wr);
+    lds_set(swl, tid + 1, wl);
+    lds_set(swr, tid, wr);
+  }
+  __syncthreads();
+  if (tid >= 2 && tid <= kX1Block - 2 && cell <= g.ie + 2) {
+    Prim7 wlb, wrb;
+    pred_face_b(a, k, 1u, lds_get(swl, tid), lds_get(swr, tid), g.dx1, F_B1I, F_M2, true,
+                ld7(a.ab + A_F1 * ps, ps, k), ld7(a.ab + A_UL1 * ps, ps, k),
+                ld7(a.ab + A_UR1 * ps, ps, k), wlb, wrb);
+    st7(a.ab + A_WL1 * ps, ps, k, wlb);
+    st7(a.ab + A_WR1 * ps, ps, k, wrb);
+  }
+}
+
+// Steps 2c-2e reversed: lanes own adjacent columns and march down a segment
+// of x2 faces (the forward k_pred_x2 layout)
+__global__ void __launch_bounds__(kX2Block) ak_pred_x2(AdjArgs a) {
+  const Inst& g = a.g;
+  const int lane = threadIdx.x;
+  const int ncb = (g.nx1 + 5 + kX2Block - 1) / kX2Block;
+  const int sg = (int)blockIdx.x / ncb, cb = (int)blockIdx.x - sg * ncb;
+  const int i = g.is - 2 + cb * kX2Block + lane;
+  if (i > g.ie + 2) return;
+  const int jf0 = g.js - 1 + sg * a.x2_seg;
+  const int jf1 = min(jf0 + a.x2_seg - 1, g.je + 2);
+  const double dtodx2 = a.dt / g.dx2;
+  const EosC eos{g.gamma, g.g1, g.g2};
+  const double* st = a.st;
+  const size_t ps = g.ps;
+  const int pitch = g.pitch;
+  auto prim_at = [&](int jj, double& bx) -> Prim7 {
+    const uint32_t k = (uint32_t)(jj * pitch + i);
+    const Cons7 u{st[F_D * ps + k], st[F_M2 * ps + k], st[F_M3 * ps + k], st[F_M1 * ps + k],
+                  st[F_E * ps + k], st[F_B3C * ps + k], st[F_B1C * ps + k]};
+    bx = st[F_B2C * ps + k];
+    return cons_to_prim(u, bx, eos);
+  };
+  double bxa, bxb, bxc;
+  Prim7 wa = prim_at(jf0 - 2, bxa);
+  Prim7 wb = prim_at(jf0 - 1, bxb);
+  Prim7 wc = prim_at(jf0, bxc);
+  Prim7 wl_cur, wr_unused;
+  plm_cell(wa, wb, wc, bxb, dtodx2, eos, wl_cur, wr_unused);
+  for (int j = jf0; j <= jf1; j++) {
+    wa = wb;
+    wb = wc;
+    bxb = bxc;
+    wc = prim_at(j + 1, bxc);
+    Prim7 wl_next, wr_cur;
+    plm_cell(wa, wb, wc, bxb, dtodx2, eos, wl_next, wr_cur);
+    const uint32_t k = (uint32_t)(j * pitch + i);
+    Prim7 wlb, wrb;
+    pred_face_b(a, k, (uint32_t)pitch, wl_cur, wr_cur, g.dx2, F_B2I, F_M1, false,
+                ld7(a.ab + A_F2 * ps, ps, k), ld7(a.ab + A_UL2 * ps, ps, k),
+                ld7(a.ab + A_UR2 * ps, ps, k), wlb, wrb);
+    st7(a.ab + A_WL2 * ps, ps, k, wlb);
+    st7(a.ab + A_WR2 * ps, ps, k, wrb);
+    wl_cur = wl_next;
+  }
+}
+
+// Steps 1a-1b / 2a-2b reversed for one cell of [js-2,je+2] x [is-2,ie+2]:
+// the PLM of cell k produced Wl at the next face and Wr at its own face.
+template <bool X1>
+__global__ void __launch_bounds__(kTileX* kTileY) ak_plm(AdjArgs a) {
+  const Inst& g = a.g;
+  int j, i;
+  if (!range_ij(g.js - 2, g.je + 2, g.is - 2, g.ie + 2, j, i)) return;
+  const size_t ps = g.ps;
+  const uint32_t stride = X1 ? 1u : (uint32_t)g.pitch;
+  const uint32_t k = (uint32_t)(j * g.pitch + i);
+  double olb[7], orb[7];
+  bool any = false;
+  {
+    const Prim7 l = ld7p(a.ab + (X1 ? A_WL1 : A_WL2) * ps, ps, k + stride);
+    const Prim7 r = ld7p(a.ab + (X1 ? A_WR1 : A_WR2) * ps, ps, k);
+    for (int n = 0; n < 7; n++) {
+      olb[n] = arr(l)[n];
+      orb[n] = arr(r)[n];
+      any = any || olb[n] != 0.0 || orb[n] != 0.0;
+    }
+  }
+  if (!any) return;
+  const EosC eos{g.gamma, g.g1, g.g2};
+  const double* st = a.st;
+  // cell state in the sweep's order and its normal field
+  const int q[7] = {F_D, X1 ? F_M1 : F_M2, X1 ? F_M2 : F_M3, X1 ? F_M3 : F_M1, F_E,
+                    X1 ? F_B2C : F_B3C, X1 ? F_B3C : F_B1C};
+  const int qx = X1 ? F_B1C : F_B2C;
+  auto cons_at = [&](uint32_t kk) {
+    return Cons7{st[q[0] * ps + kk], st[q[1] * ps + kk], st[q[2] * ps + kk], st[q[3] * ps + kk],
+                 st[q[4] * ps + kk], st[q[5] * ps + kk], st[q[6] * ps + kk]};
+  };
+  const uint32_t kc[3] = {k - stride, k, k + stride};
+  Prim7 W[3];
+  for (int c = 0; c < 3; c++) W[c] = cons_to_prim(cons_at(kc[c]), st[qx * ps + kc[c]], eos);
+  PlmF P;
+  P.run(W[0], W[1], W[2], st[qx * ps + k], a.dt / (X1 ? g.dx1 : g.dx2), eos);
+  double wb[3][7];
+  for (int c = 0; c < 3; c++)
+    for (int n = 0; n < 7; n++) wb[c][n] = 0.0;
+  double bxcb = 0.0;
+  P.back(olb, orb, wb[0], wb[1], wb[2], bxcb);
+  for (int c = 0; c < 3; c++) {
+    Cons7 ub = zero_cons();
+    double bxb = (c == 1) ? bxcb : 0.0;
+    const Prim7 Wb{wb[c][0], wb[c][1], wb[c][2], wb[c][3], wb[c][4], wb[c][5], wb[c][6]};
+    cons_to_prim_b(cons_at(kc[c]), st[qx * ps + kc[c]], eos, Wb, ub, bxb);
+    for (int n = 0; n < 7; n++) ADD(a.ain + q[n] * ps, kc[c], arr(ub)[n]);
+    ADD(a.ain + qx * ps, kc[c], bxb);
+  }
+}
+
+}  // namespace adjk
+}  // namespace athmi
+
+using namespace athmi;
+using namespace athmi::adjk;
+
+extern "C" hipError_t athmi_adjoint_launch(const AdjArgs* ap, int kernel, hipStream_t s) {
+  const AdjArgs& a = *ap;
+  const Inst& g = a.g;
+  const dim3 b2(kTileX, kTileY);
+  auto grid2 = [&](int j0, int j1, int i0, int i1) {
+    return dim3((unsigned)cdiv(i1 - i0 + 1, kTileX), (unsigned)cdiv(j1 - j0 + 1, kTileY));
+  };
+  switch (kernel) {
+    case AK_FOLD_OUT:
+    case AK_FOLD_IN:
+      hipLaunchKernelGGL(ak_fold, grid2(0, g.sy - 1, 0, g.sx - 1), b2, 0, s, a,
+                         kernel == AK_FOLD_IN ? 1 : 0);
+      break;
+    case AK_ZERO:
+      hipLaunchKernelGGL(ak_zero, dim3(2048), dim3(256), 0, s, a);
+      break;
+    case AK_UPDATE:
+      hipLaunchKernelGGL(ak_update, grid2(g.js, g.je + 1, g.is, g.ie + 1), b2, 0, s, a);
+      break;
+    case AK_CORNER_C:
+      hipLaunchKernelGGL(ak_corner<false>, grid2(g.js, g.je + 1, g.is, g.ie + 1), b2, 0, s, a);
+      break;
+    case AK_RIEMANN_C:
+      hipLaunchKernelGGL(ak_riemann_c, grid2(g.js - 1, g.je + 1, g.is - 1, g.ie + 1), b2, 0, s, a);
+      break;
+    case AK_HALF:
+      hipLaunchKernelGGL(ak_half, grid2(g.js - 1, g.je + 1, g.is - 1, g.ie + 1), b2, 0, s, a);
+      break;
+    case AK_TRANSVERSE:
+      hipLaunchKernelGGL(ak_transverse, grid2(g.js - 1, g.je + 2, g.is - 1, g.ie + 2), b2, 0, s, a);
+      break;
+    case AK_CORNER_P:
+      hipLaunchKernelGGL(ak_corner<true>, grid2(g.js - 1, g.je + 2, g.is - 1, g.ie + 2), b2, 0, s, a);
+      break;
+    case AK_EMF_CC:
+      hipLaunchKernelGGL(ak_emf_cc, grid2(g.js - 2, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);
+      break;
+    case AK_PRED_X1:
+      hipLaunchKernelGGL(ak_pred_x1, dim3((unsigned)((g.nx2 + 4) * cdiv(g.nx1 + 4, kX1Faces))),
+                         dim3(kX1Block), 0, s, a);
+      break;
+    case AK_PLM_X1:
+      hipLaunchKernelGGL(ak_plm<true>, grid2(g.js - 2, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);
+      break;
+    case AK_PRED_X2:
+      hipLaunchKernelGGL(ak_pred_x2,
+                         dim3((unsigned)(cdiv(g.nx1 + 5, kX2Block) * cdiv(g.nx2 + 4, a.x2_seg))),
+                         dim3(kX2Block), 0, s, a);
+      break;
+    case AK_PLM_X2:
+      hipLaunchKernelGGL(ak_plm<false>, grid2(g.js - 2, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}

@@ -9,9 +9,32 @@
 #ifndef ATHMI_ADJOINT_H
 #define ATHMI_ADJOINT_H
 
+#include "athmi.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* A rollout of one instance: set_bvals, then nsteps times integrate_2d with
+ * the fixed dt followed by set_bvals (main.c:221-318 with new_dt bypassed).
+ * The state is held as dense device arrays [NSTATE=11][Nx2+8][Nx1+8] in the
+ * order d, M1, M2, M3, E, B1c, B2c, B3c, B1i, B2i, B3i. The forward pass keeps
+ * a checkpoint every `every` steps; the backward pass recomputes each segment
+ * from its checkpoint and reverses it step by step. Requires ATHMI_EXACT
+ * arithmetic (the adjoint is linearised at the branches of that forward
+ * pass) and Nx1, Nx2 >= 4. Ghost values of state0 are ignored (set_bvals
+ * overwrites them), so their gradient is zero. */
+typedef struct athmi_adjoint athmi_adjoint;
+int athmi_adjoint_create(const athmi_desc *desc, double dt, int nsteps, int every,
+                         athmi_adjoint **out);
+int athmi_adjoint_destroy(athmi_adjoint *h);
+int athmi_adjoint_set_stream(athmi_adjoint *h, void *hip_stream);
+/* state0, stateN: device pointers (dense layout above). Both calls run on the
+ * handle's stream and synchronise it before returning. */
+int athmi_adjoint_forward(athmi_adjoint *h, const double *state0, double *stateN);
+/* gradN = dL/dstateN, grad0 = dL/dstate0 (device pointers, dense layout);
+ * valid after athmi_adjoint_forward with the same state0 */
+int athmi_adjoint_backward(athmi_adjoint *h, const double *gradN, double *grad0);
 
 /* Per-routine vector-Jacobian products on the host (same source as the
  * adjoint kernels). Vectors use the Cons1D / Prim1D member order

@@ -4,10 +4,13 @@
 // flux_hlle.c, convert_var.c, lr_states_plm.c, __integrate_2d__.c,
 // set_bvals.c). The adjoint kernels are checked against it by dot products:
 // <ybar, J xdot> from here equals <J^T ybar, xdot> from the adjoint.
+#include <omp.h>
+
 #include <cstring>
 #include <vector>
 
 #include "dual.hpp"
+#include "counted.hpp"
 #include "athena_oracle.hpp"
 
 using namespace ora;
@@ -127,6 +130,34 @@ void ora_tlm_rollout(int nx1, int nx2, double x1min, double x1max, double x2min,
       dout[q * np + k] = p[k].t;
     }
   }
+}
+
+// FP64 operations of nsteps calls of integrate_2d on a state (11 planes, as
+// above; ghosts as given), counted on the reference algorithm as restated:
+// ops[0] add/sub, ops[1] mul, ops[2] div, ops[3] sqrt. The max/min selections
+// and comparisons are not counted.
+void ora_count_ops(int nx1, int nx2, double x1min, double x1max, double x2min, double x2max,
+                   double gamma, int flux, double dt, int nsteps, const double* state,
+                   unsigned long long* ops) {
+  Grid<Counted> G;
+  init_grid(G, nx1, nx2, x1min, x1max, x2min, x2max, gamma, 0.4);
+  G.flux = flux;
+  const size_t np = (size_t)G.sx * G.sy;
+  for (int q = 0; q < 11; q++) {
+    std::vector<Counted>& p = *G.plane(q);
+    for (size_t k = 0; k < np; k++) p[k] = Counted(state[q * np + k]);
+  }
+  Scratch<Counted> S;
+  G.dt = dt;
+  const int threads = omp_get_max_threads();
+  omp_set_num_threads(1);  // the counters are per thread
+  op_count() = OpCount();
+  for (int s = 0; s < nsteps; s++) integrate_2d(G, S);
+  omp_set_num_threads(threads);
+  ops[0] = op_count().add;
+  ops[1] = op_count().mul;
+  ops[2] = op_count().div;
+  ops[3] = op_count().sqrt;
 }
 
 }  // extern "C"

@@ -208,3 +208,32 @@ def test_tlm_matches_finite_differences():
                        bx, bd, p(f), p(fd))
         fdiff = (F(eps) - F(-eps)) / (2 * eps)
         assert np.allclose(fd, fdiff, rtol=1e-5, atol=1e-6 * (1 + np.abs(fd).max())), k
+
+
+def test_tlm_rollout_matches_finite_differences():
+    """Pins the rollout TLM (integrate_2d + set_bvals on dual numbers) against
+    central differences of the same rollout in doubles."""
+    from adjoint_cases import rollout_state, stable_dt
+    O = oracle_lib()
+    n1, n2, nsteps = 12, 10, 3
+    s0 = rollout_state(5, n1, n2)
+    dt = stable_dt(s0, n1, n2)
+    bc = (ctypes.c_int * 4)(4, 4, 2, 2)
+    rng = np.random.default_rng(6)
+    xd = rng.normal(size=s0.shape)
+    yb = rng.normal(size=s0.shape)
+
+    def run(state, tangent):
+        out, dout = np.zeros_like(state), np.zeros_like(state)
+        O.ora_tlm_rollout(n1, n2, 0.0, 1.0, 0.0, 1.0, GAMMA, bc, 0, dt, nsteps,
+                          p(np.ascontiguousarray(state)), p(np.ascontiguousarray(tangent)),
+                          p(out), p(dout))
+        return out, dout
+
+    _, dout = run(s0, xd)
+    eps = 1e-6
+    fp, _ = run(s0 + eps * xd, np.zeros_like(s0))
+    fm, _ = run(s0 - eps * xd, np.zeros_like(s0))
+    fd = yb.ravel() @ ((fp - fm) / (2 * eps)).ravel()
+    tl = yb.ravel() @ dout.ravel()
+    assert abs(fd - tl) <= 1e-6 * abs(tl), (fd, tl)
