@@ -65,7 +65,8 @@ SIGNATURES = {
     "athmi_sync": (_I, [_P]),
     "athmi_set_shape": (_I, [_P, _I, _I]),
     "athmi_use_graph": (_I, [_P, _I]),
-    "athmi_autotune": (_I, [_P, _I, _IP, _IP, _DP]),
+    "athmi_autotune": (_I, [_P, _I, _IP, _IP, _IP, _DP]),
+    "athmi_set_tile_order": (_I, [_P, _I]),
     "athmi_device_state": (_I, [_P, _I, ctypes.POINTER(_P), _IP]),
     "athmi_pgen_blast": (_I, [_P, _I, _D, _D, _D, _D, _D, _D, _D]),
     "athmi_pgen_field_loop": (_I, [_P, _I, _D, _D, _D, _I]),
@@ -249,11 +250,14 @@ class Grid:
 
     def autotune(self, steps=2):
         """Pick the fastest launch shape for this batch (state is restored).
-        Returns (x2_seg, persist, ms_per_step)."""
-        a, b, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        Returns (x2_seg, persist, xcd_remap, ms_per_step)."""
+        a, b, c, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         _check(lib().athmi_autotune(self._h, steps, ctypes.byref(a), ctypes.byref(b),
-                                    ctypes.byref(ms)), "autotune")
-        return a.value, b.value, ms.value
+                                    ctypes.byref(c), ctypes.byref(ms)), "autotune")
+        return a.value, b.value, c.value, ms.value
+
+    def set_tile_order(self, xcd_remap=1):
+        _check(lib().athmi_set_tile_order(self._h, int(xcd_remap)), "set_tile_order")
 
     def use_graph(self, on=True):
         _check(lib().athmi_use_graph(self._h, 1 if on else 0), "use_graph")

@@ -11,11 +11,17 @@ are float64 CUDA tensors of shape (11, Nx2+8, Nx1+8) holding the Grid fields
     sN = rollout(s0, h)                 # differentiable
     loss = 0.5 * ((sN[0] - target) ** 2)[interior].sum()
     loss.backward()                     # s0.grad via the adjoint
+
+Import this module (it imports torch and initialises its HIP runtime) before
+the first use of libathmi.so in a process: torch ships its own HIP runtime
+under the same soname as the one the library links, and whichever is loaded
+first serves the whole process.
 """
 import ctypes
 import math
 
 import numpy as np
+import torch
 
 from . import AthmiError, _Desc, _check, lib, FLUXES, VARIANTS
 
@@ -23,6 +29,9 @@ _D, _I, _P = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
 
 
 def _bind():
+    if not torch.cuda.is_available():
+        raise AthmiError("torch sees no GPU: import athena_cuda_amd.adjoint (which imports "
+                         "torch) before the first use of libathmi.so in this process")
     L = lib()
     if not getattr(L, "_adjoint_bound", False):
         L.athmi_adjoint_create.argtypes = [ctypes.POINTER(_Desc), _D, _I, _I,

@@ -19,7 +19,8 @@
 namespace {
 
 // forward scratch of one step: predictor, predictor corner emf, corrector fluxes
-const int kSeqForwardScratch[] = {SK_BEGIN_ONCE, SK_PRED_X1, SK_PRED_X2, SK_CORNER_P, SK_CORR};
+const int kSeqForwardScratch[] = {SK_BEGIN_ONCE, SK_PRED_X1, SK_PRED_X2, SK_CORNER_P, SK_CORR,
+                                  SK_CORR2};
 const int kSeqReverse[] = {AK_FOLD_OUT,  AK_ZERO,       AK_UPDATE,   AK_CORNER_C, AK_RIEMANN_C,
                            AK_HALF,      AK_TRANSVERSE, AK_CORNER_P, AK_EMF_CC,   AK_PRED_X1,
                            AK_PLM_X1,    AK_PRED_X2,    AK_PLM_X2};

@@ -19,7 +19,6 @@
 namespace athmi {
 namespace adj {
 
-#define ATHMI_HDM __host__ __device__ __forceinline__  // member functions
 
 ATHMI_HD double sgnabs(double x) { return (x < 0.0) ? -1.0 : 1.0; }  // d|x|/dx
 // adjoint of y = sqrt(x): xb += yb / (2 y), zero at y == 0

@@ -59,6 +59,7 @@ int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
   g->bl.ncu = prop.multiProcessorCount;
   g->bl.shape.x2_seg = 32;
   g->bl.shape.persist = 0;
+  g->bl.shape.xcd_remap = 1;
   // device memory: per instance 2*NSTATE state planes + NSCRATCH scratch planes
   std::vector<size_t> off(n);
   size_t total = 0;
@@ -386,7 +387,8 @@ int athmi_set_shape(athmi_grid* g, int x2_seg, int persist) {
 // occupancy) runs the same `steps` main-loop steps from the same state, timed
 // with HIP events on the handle's stream through the step graph; the state
 // and clocks are restored after each candidate and the fastest shape is kept.
-int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, double* ms_per_step) {
+int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd_remap,
+                   double* ms_per_step) {
   if (!g || steps <= 0) return fail(ATHMI_EINVAL, "athmi_autotune: bad arguments");
   static const int kSeg[] = {16, 32, 64, 128};
   static const int kPersist[] = {0, 1, 2, 4, 8};
@@ -422,10 +424,12 @@ int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, double* 
   LaunchShape best_shape = old;
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = fail(ATHMI_EHIP, "athmi_autotune: event creation failed");
+  for (int c = 0; !rc && c < 2; c++)
   for (int a = 0; !rc && a < (int)(sizeof(kSeg) / sizeof(int)); a++) {
     for (int b = 0; !rc && b < (int)(sizeof(kPersist) / sizeof(int)); b++) {
       g->bl.shape.x2_seg = kSeg[a];
       g->bl.shape.persist = kPersist[b];
+      g->bl.shape.xcd_remap = c;
       if ((rc = g->refresh_tiles())) break;
       if ((rc = athmi_step(g))) break;  // capture + first replay, untimed
       if ((rc = copy_state(false))) break;
@@ -456,8 +460,16 @@ int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, double* 
   if (rc2) return rc2;
   if (x2_seg) *x2_seg = best_shape.x2_seg;
   if (persist) *persist = best_shape.persist;
+  if (xcd_remap) *xcd_remap = best_shape.xcd_remap;
   if (ms_per_step) *ms_per_step = best;
   return ATHMI_OK;
+}
+
+int athmi_set_tile_order(athmi_grid* g, int xcd_remap) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->bl.shape.xcd_remap = xcd_remap ? 1 : 0;
+  return g->refresh_tiles();
 }
 
 int athmi_use_graph(athmi_grid* g, int on) {

@@ -16,6 +16,7 @@
 namespace athmi {
 
 #define ATHMI_HD static __host__ __device__ __forceinline__
+#define ATHMI_HDM __host__ __device__ __forceinline__  // member functions
 
 constexpr double kTiny = 1.0e-20;  // defs.h:103 TINY_NUMBER
 

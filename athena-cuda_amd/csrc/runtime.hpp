@@ -36,14 +36,15 @@ inline int fail(int code, const std::string& msg) {
 typedef hipError_t (*LaunchFn)(const void*, int, hipStream_t);
 
 // kernel sequences of the reference's routines
-const int kSeqStep[] = {SK_BEGIN_STEP, SK_PRED_X1,  SK_PRED_X2, SK_CORNER_P, SK_CORR,
-                        SK_CORNER_C,   SK_FINAL,    SK_GHOST_NEXT, SK_END_STEP};
-const int kSeqIntegrate[] = {SK_BEGIN_ONCE, SK_PRED_X1, SK_PRED_X2,     SK_CORNER_P, SK_CORR,
-                             SK_CORNER_C,   SK_FINAL,   SK_GHOST_CARRY, SK_END_ONCE};
+const int kSeqStep[] = {SK_BEGIN_STEP, SK_PRED_X1, SK_PRED_X2, SK_CORNER_P,   SK_CORR,
+                        SK_CORR2,      SK_CORNER_C, SK_FINAL,  SK_GHOST_NEXT, SK_END_STEP};
+const int kSeqIntegrate[] = {SK_BEGIN_ONCE, SK_PRED_X1,  SK_PRED_X2, SK_CORNER_P,    SK_CORR,
+                             SK_CORR2,      SK_CORNER_C, SK_FINAL,   SK_GHOST_CARRY, SK_END_ONCE};
 const int kSeqNewDt[] = {SK_CLEAR, SK_DTI, SK_DT_FINAL};
 const int kSeqDivb[] = {SK_CLEAR, SK_DIVB};
-const char* const kStepNames[] = {"k_begin", "k_pred_x1", "k_pred_x2", "k_corner<pred>", "k_corr",
-                            "k_corner<corr>", "k_final", "k_ghost", "k_end"};
+const char* const kStepNames[] = {"k_begin",   "k_pred_x1",      "k_pred_x2", "k_corner<pred>",
+                                  "k_corr<1>", "k_corr<2>",      "k_corner<corr>", "k_final",
+                                  "k_ghost",   "k_end"};
 constexpr int kStepLen = sizeof(kSeqStep) / sizeof(int);
 
 inline size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }

@@ -31,6 +31,17 @@ namespace ATHMI_VARIANT {
 
 __device__ __forceinline__ int dcdiv(int a, int b) { return (a + b - 1) / b; }
 
+// The hardware deals workgroups to the 8 XCDs round-robin (workgroup b to XCD
+// b % 8). With remap, tile t of n is renumbered so that each XCD gets one
+// contiguous run of tiles: row neighbours and the rows above and below then
+// share that XCD's L2. Any bijection is correct; this one only changes
+// which L2 serves which tile.
+__device__ __forceinline__ int xcd_tile(int t, int n, int remap) {
+  if (!remap) return t;
+  const int x = t & 7, q = t >> 3, per = n >> 3, rem = n & 7;
+  return x < rem ? x * (per + 1) + q : rem * (per + 1) + (x - rem) * per + q;
+}
+
 // instance owning global tile t of kernel kind `kind` (tile0 ascending)
 __device__ __forceinline__ int find_inst(const Inst* __restrict__ in, int n, int kind, int t) {
   int lo = 0, hi = n - 1;
@@ -106,12 +117,13 @@ __device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double 
 // the characteristic PLM (lr_states_plm.c:95-245); each face then gets the
 // 1/2 dt MHD source on By, prim->cons and the Riemann flux.
 __global__ void __launch_bounds__(kX1Block) k_pred_x1(const Inst* __restrict__ insts, int ninst,
-                                                      int ntiles) {
+                                                      int ntiles, int remap) {
   __shared__ double sw[7][kX1Block];
   __shared__ double swl[7][kX1Block];
   __shared__ double swr[7][kX1Block];
   const int tid = threadIdx.x;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
+    const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, K_PRED_X1, t)];
     const Clock* clk = g.clk;
     if (!clk->active) continue;
@@ -185,11 +197,12 @@ __global__ void __launch_bounds__(kX1Block) k_pred_x1(const Inst* __restrict__ i
 // columns (coalesced rows); each lane marches down a segment of faces keeping
 // the three-cell PLM stencil and the pending left state in registers.
 __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ insts, int ninst,
-                                                      int ntiles, int seg_len) {
+                                                      int ntiles, int seg_len, int remap) {
   __shared__ double sxl[7][kX2Block];
   __shared__ double sxr[7][kX2Block];
   const int lane = threadIdx.x;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
+    const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, K_PRED_X2, t)];
     const Clock* clk = g.clk;
     if (!clk->active) continue;
@@ -265,9 +278,10 @@ __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ i
 // fluxes; otherwise Step 9 over [js,je+1]x[is,ie+1], the corners Step 9 uses.
 template <bool PRED>
 __global__ void __launch_bounds__(kTileX* kTileY) k_corner(const Inst* __restrict__ insts,
-                                                            int ninst, int ntiles) {
+                                                            int ninst, int ntiles, int remap) {
   const int kind = PRED ? K_CORNER_P : K_CORNER_C;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
+    const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, kind, t)];
     if (!g.clk->active) continue;
     const int local = t - g.tile0[kind];
@@ -316,9 +330,14 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_corner(const Inst* __restric
 //  - x2-face states: transverse x1-flux gradients + MHD sources (:343-393),
 //    then the corrector x2 flux (:455-464) for j >= js
 //  - d, M1, M2 at t^{n+1/2} and the cell-centred emf3 there (:402-431)
+// PART 1: the x1 faces (Steps 5, 8b); PART 2: the x2 faces and Step 7 (6, 8c, 7).
+// Split so that each thread evaluates one Riemann flux: the loads of the second
+// face no longer compete for registers with the first face's flux.
+template <int PART>
 __global__ void __launch_bounds__(kTileX* kTileY) k_corr(const Inst* __restrict__ insts,
-                                                          int ninst, int ntiles) {
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+                                                             int ninst, int ntiles, int remap) {
+  for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
+    const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, K_CORR, t)];
     const Clock* clk = g.clk;
     if (!clk->active) continue;
@@ -344,7 +363,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_corr(const Inst* __restrict_
     auto b2h = [&](uint32_t kk) { return LD(B2i, kk) + hdtodx1 * (LD(emf, kk + 1) - LD(emf, kk)); };
     const double* F1 = sc + S_F1 * ps;
     const double* F2 = sc + S_F2 * ps;
-    if (i >= g.is) {
+    if (PART == 1 && i >= g.is) {
       // corrected x1-face states (Steps 5a, 5b) at the half-step face field;
       // rebuilt from memory for the HLLE fallback
       auto build = [&](Cons7& L, Cons7& R, Prim7& wl, Prim7& wr, double& bx) {
@@ -395,6 +414,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_corr(const Inst* __restrict_
       build(L, R, wl, wr, bx);
       put7(sc + S_F1C * ps, ps, k, riemann_reload(g.flux, L, R, wl, wr, bx, eos, build));
     }
+    if (PART == 1) continue;
     if (j >= g.js) {
       // corrected x2-face states (Steps 6a, 6b) at the half-step face field
       auto build = [&](Cons7& L, Cons7& R, Prim7& wl, Prim7& wr, double& bx) {
@@ -477,9 +497,10 @@ __device__ __forceinline__ double inverse_dt(const Inst& g, double d, double M1,
 // ping-pong buffer, and the new_dt maximum of the updated interior. The B2c
 // update of 11a and the B1c update of 11b are dropped: Step 13 overwrites them.
 __global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict__ insts,
-                                                           int ninst, int ntiles) {
+                                                           int ninst, int ntiles, int remap) {
   __shared__ double red[4];
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
+    const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, K_FINAL, t)];
     Clock* clk = g.clk;
     if (!clk->active) continue;
@@ -553,8 +574,10 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict
 // unchanged into the other one (an in-place integrate_2d leaves them as they
 // were). respect_active = 0 runs regardless of the clock.
 __global__ void __launch_bounds__(kGhostBlock) k_ghost(const Inst* __restrict__ insts, int ninst,
-                                                       int ntiles, int which, int respect_active) {
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+                                                       int ntiles, int which, int respect_active,
+                                                       int remap) {
+  for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
+    const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, K_GHOST, t)];
     const Clock* clk = g.clk;
     if (respect_active && !clk->active) continue;
@@ -636,9 +659,10 @@ __global__ void __launch_bounds__(kDtSlots) k_end(const Inst* __restrict__ insts
 
 // standalone new_dt (new_dt.c:25-103) on the current state
 __global__ void __launch_bounds__(kTileX* kTileY) k_dti(const Inst* __restrict__ insts, int ninst,
-                                                         int ntiles) {
+                                                         int ntiles, int remap) {
   __shared__ double red[4];
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
+    const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, K_INTERIOR, t)];
     Clock* clk = g.clk;
     const int local = t - g.tile0[K_INTERIOR];
@@ -677,9 +701,10 @@ __global__ void __launch_bounds__(256) k_clear_slots(const Inst* __restrict__ in
 
 // max over the interior of |div B| = |dB1/dx1 + dB2/dx2| on the current state
 __global__ void __launch_bounds__(kTileX* kTileY) k_divb(const Inst* __restrict__ insts, int ninst,
-                                                          int ntiles) {
+                                                          int ntiles, int remap) {
   __shared__ double red[4];
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
+    const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, K_INTERIOR, t)];
     Clock* clk = g.clk;
     const int local = t - g.tile0[K_INTERIOR];
@@ -733,34 +758,40 @@ hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
       break;
     case SK_PRED_X1:
       hipLaunchKernelGGL(k_pred_x1, dim3(grid_of(b, K_PRED_X1, 4)), dim3(kX1Block), 0, s,
-                         b.insts, b.ninst, b.ntiles[K_PRED_X1]);
+                         b.insts, b.ninst, b.ntiles[K_PRED_X1], b.shape.xcd_remap);
       break;
     case SK_PRED_X2:
       hipLaunchKernelGGL(k_pred_x2, dim3(grid_of(b, K_PRED_X2, 16)), dim3(kX2Block), 0, s,
-                         b.insts, b.ninst, b.ntiles[K_PRED_X2], b.shape.x2_seg);
+                         b.insts, b.ninst, b.ntiles[K_PRED_X2], b.shape.x2_seg,
+                         b.shape.xcd_remap);
       break;
     case SK_CORNER_P:
       hipLaunchKernelGGL(k_corner<true>, dim3(grid_of(b, K_CORNER_P, 8)), b2, 0, s, b.insts,
-                         b.ninst, b.ntiles[K_CORNER_P]);
+                         b.ninst, b.ntiles[K_CORNER_P], b.shape.xcd_remap);
       break;
     case SK_CORR:
-      hipLaunchKernelGGL(k_corr, dim3(grid_of(b, K_CORR, 4)), b2, 0, s, b.insts, b.ninst,
-                         b.ntiles[K_CORR]);
+      hipLaunchKernelGGL(k_corr<1>, dim3(grid_of(b, K_CORR, 4)), b2, 0, s, b.insts, b.ninst,
+                         b.ntiles[K_CORR], b.shape.xcd_remap);
+      break;
+    case SK_CORR2:
+      hipLaunchKernelGGL(k_corr<2>, dim3(grid_of(b, K_CORR, 4)), b2, 0, s, b.insts, b.ninst,
+                         b.ntiles[K_CORR], b.shape.xcd_remap);
       break;
     case SK_CORNER_C:
       hipLaunchKernelGGL(k_corner<false>, dim3(grid_of(b, K_CORNER_C, 8)), b2, 0, s, b.insts,
-                         b.ninst, b.ntiles[K_CORNER_C]);
+                         b.ninst, b.ntiles[K_CORNER_C], b.shape.xcd_remap);
       break;
     case SK_FINAL:
       hipLaunchKernelGGL(k_final, dim3(grid_of(b, K_FINAL, 8)), b2, 0, s, b.insts, b.ninst,
-                         b.ntiles[K_FINAL]);
+                         b.ntiles[K_FINAL], b.shape.xcd_remap);
       break;
     case SK_GHOST_CUR:
     case SK_GHOST_NEXT:
     case SK_GHOST_CARRY: {
       const int which = kernel == SK_GHOST_CUR ? 0 : (kernel == SK_GHOST_NEXT ? 1 : 2);
       hipLaunchKernelGGL(k_ghost, dim3(grid_of(b, K_GHOST, 8)), dim3(kGhostBlock), 0, s, b.insts,
-                         b.ninst, b.ntiles[K_GHOST], which, kernel == SK_GHOST_NEXT ? 1 : 0);
+                         b.ninst, b.ntiles[K_GHOST], which, kernel == SK_GHOST_NEXT ? 1 : 0,
+                         b.shape.xcd_remap);
       break;
     }
     case SK_END_STEP:
@@ -773,14 +804,14 @@ hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
       break;
     case SK_DTI:
       hipLaunchKernelGGL(k_dti, dim3(grid_of(b, K_INTERIOR, 8)), b2, 0, s, b.insts, b.ninst,
-                         b.ntiles[K_INTERIOR]);
+                         b.ntiles[K_INTERIOR], b.shape.xcd_remap);
       break;
     case SK_DT_FINAL:
       hipLaunchKernelGGL(k_dt_final, dim3(b.ninst), dim3(kDtSlots), 0, s, b.insts);
       break;
     case SK_DIVB:
       hipLaunchKernelGGL(k_divb, dim3(grid_of(b, K_INTERIOR, 8)), b2, 0, s, b.insts, b.ninst,
-                         b.ntiles[K_INTERIOR]);
+                         b.ntiles[K_INTERIOR], b.shape.xcd_remap);
       break;
     default:
       return hipErrorInvalidValue;

@@ -95,6 +95,7 @@ inline int tiles_of(int k, int nx1, int nx2, int x2seg) {
 struct LaunchShape {
   int x2_seg;        // faces marched per thread in the x2 predictor
   int persist;       // 0: one tile per workgroup; >0: workgroups per CU of a persistent grid
+  int xcd_remap;     // 1: contiguous runs of tiles per XCD (L2 locality), 0: launch order
 };
 
 // Everything a launcher needs for one batch (device descriptor array, tile
@@ -114,7 +115,8 @@ enum StepKernel {
   SK_PRED_X1,
   SK_PRED_X2,
   SK_CORNER_P,
-  SK_CORR,
+  SK_CORR,            // corrector, x1 faces
+  SK_CORR2,           // corrector, x2 faces and emf3 at t^{n+1/2}
   SK_CORNER_C,
   SK_FINAL,
   SK_GHOST_CUR,       // set_bvals on the current buffer

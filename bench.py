@@ -38,7 +38,11 @@ KERNEL_IO = {
     "k_pred_x1": (9, 22, lambda a, b: (a + 4) * (b + 4)),       # U, B1i -> Ul,Ur,F (21) + emf3_cc
     "k_pred_x2": (9, 21, lambda a, b: (a + 5) * (b + 4)),       # U, B2i -> Ul,Ur,F (21)
     "k_corner<pred>": (5, 1, lambda a, b: (a + 4) * (b + 4)),
-    "k_corr": (47, 15, lambda a, b: (a + 3) * (b + 3)),         # faces, fluxes, U, B, emf -> 2x7 + 1
+    # corrector x1 faces: Ul1, Ur1, 6 x2-flux components, B1i, B1c..B3c, M3, d, emf -> F1c
+    "k_corr<1>": (27, 7, lambda a, b: (a + 2) * (b + 3)),
+    # corrector x2 faces + Step 7: Ul2, Ur2, 6 + 3 flux components, B1i, B2i, cell fields,
+    # emf -> F2c, emf3 at t^{n+1/2}
+    "k_corr<2>": (33, 8, lambda a, b: (a + 3) * (b + 3)),
     "k_corner<corr>": (5, 1, lambda a, b: (a + 2) * (b + 2)),
     "k_final": (26, 11, lambda a, b: (a + 1) * (b + 1)),        # U, B, 14 fluxes, emf -> U, B
     "k_ghost": (11, 11, lambda a, b: 8 * (a + 8) + 8 * (b + 1)),
@@ -113,7 +117,8 @@ def batched_leg(pkg, args, plain_rate):
     shape = None
     if not args.no_tune:
         shape = gb.autotune(2)
-        log(f"[bench] batch shape x2_seg={shape[0]} persist={shape[1]} ({shape[2]:.3f} ms/step)")
+        log(f"[bench] batch shape x2_seg={shape[0]} persist={shape[1]} xcd_remap={shape[2]} "
+            f"({shape[3]:.3f} ms/step)")
     for _ in range(args.batch_warmup):
         gb.step()
     gb.sync()
@@ -135,7 +140,110 @@ def batched_leg(pkg, args, plain_rate):
             "workload": "seeded uneven blast ensemble (seed 0): nx, ny in {64..1024 step 32}, "
                         "off-centre blasts, Roe even / HLLE odd, per-instance CFL dt, "
                         "one HIP graph per step",
-            "shape": None if shape is None else {"x2_seg": shape[0], "persist": shape[1]}}
+            "shape": None if shape is None else {"x2_seg": shape[0], "persist": shape[1],
+                                                 "xcd_remap": shape[2]}}
+
+
+def adjoint_leg(pkg, args):
+    """Differentiation form: loss = 0.5 ||rho(T) - rho_target||^2 after N
+    fixed-dt steps at 512^2 (checkpoint every 16). Cost ratio = (forward with
+    checkpoints + reverse with recomputation) / plain forward of N steps."""
+    import importlib
+    import numpy as np
+    import torch
+    A = importlib.import_module(pkg.__name__ + ".adjoint")
+    n, N, every, dt = args.adj_size, args.adj_steps, 16, 2e-4
+    dev = torch.device("cuda:0")
+    desc = A.sensitivity_desc(n)
+    s0_np, _ = A.sensitivity_state(1, n)
+    _, target = A.sensitivity_state(2, n)
+    s0 = torch.from_numpy(s0_np).to(dev)
+    tgt = torch.from_numpy(target).to(dev)
+    inner = (slice(4, n + 4), slice(4, n + 4))
+    log(f"[bench] adjoint: {n}x{n}, N={N}, checkpoint every {every}")
+    # plain forward: the same rollout through the plain step (no checkpoints)
+    g = pkg.Grid(desc, variant="exact")
+    gas = np.ascontiguousarray(np.moveaxis(s0_np[:8], 0, 2))
+    g.upload(0, gas, s0_np[8], s0_np[9], s0_np[10])
+    g.set_clock(0, dt, 0.0, 0)
+    g.set_bvals()
+    for _ in range(3):
+        g.integrate()
+        g.set_bvals()
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(N):
+        g.integrate()
+        g.set_bvals()
+    g.sync()
+    t_plain = time.perf_counter() - t0
+    g.close()
+    h = A.Rollout(desc, dt, N, every)
+    x = s0.clone().requires_grad_(True)
+    sN = A.rollout(x, h)  # warm-up of both passes
+    (0.5 * ((sN[0][inner] - tgt) ** 2).sum()).backward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sN = h.forward(s0)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    gN = torch.zeros_like(sN)
+    gN[0][inner] = sN[0][inner] - tgt
+    t0 = time.perf_counter()
+    g0 = h.backward(gN)
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    ratio = (t_fwd + t_bwd) / t_plain
+    log(f"[bench] adjoint: plain {1e3 * t_plain:.1f} ms, forward+ckpt {1e3 * t_fwd:.1f} ms, "
+        f"reverse {1e3 * t_bwd:.1f} ms, ratio {ratio:.2f}")
+
+    def fd_check(state, tg, handle, dirs, eps=1e-6):
+        """max relative |FD - adjoint| over the directions (central, eps)."""
+        t = torch.from_numpy(tg).to(dev)
+
+        def loss_of(st):
+            return 0.5 * ((handle.forward(st)[0][inner] - t) ** 2).sum().item()
+
+        xx = state.clone().requires_grad_(True)
+        (0.5 * ((A.rollout(xx, handle)[0][inner] - t) ** 2).sum()).backward()
+        rels = []
+        for v in dirs:
+            v = torch.from_numpy(v).to(dev)
+            fd = (loss_of(state + eps * v) - loss_of(state - eps * v)) / (2 * eps)
+            ad = (xx.grad * v).sum().item()
+            rels.append(abs(fd - ad) / abs(ad))
+        return max(rels)
+
+    rng = np.random.default_rng(7)
+    dens = []
+    for _ in range(args.adj_dirs):
+        v = np.zeros_like(s0_np)
+        v[0][inner] = rng.normal(size=(n, n))
+        dens.append(v)
+    rel_dens = fd_check(s0, target, h, dens)
+    log(f"[bench] adjoint FD (density directions, {len(dens)}): max rel {rel_dens:.2e}")
+    h.close()
+    import sys as _sys
+    _sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from adjoint_cases import rollout_state, stable_dt
+    s_sm = rollout_state(21, n, n)
+    h2 = A.Rollout(dict(desc), stable_dt(s_sm, n, n), args.adj_smooth_steps, every)
+    dirs = [rng.normal(size=s_sm.shape) for _ in range(args.adj_smooth_dirs)]
+    s_sm_t = torch.from_numpy(s_sm).to(dev)
+    tg_sm = rollout_state(22, n, n)[0][inner]
+    rel_sm = fd_check(s_sm_t, tg_sm, h2, dirs)
+    rel_sm8 = fd_check(s_sm_t, tg_sm, h2, dirs, eps=1e-8)
+    log(f"[bench] adjoint FD (smooth state, all fields, {len(dirs)}): max rel {rel_sm:.2e} "
+        f"(eps 1e-6), {rel_sm8:.2e} (eps 1e-8)")
+    h2.close()
+    return {"cost_ratio": ratio, "plain_forward_ms": 1e3 * t_plain,
+            "forward_with_checkpoints_ms": 1e3 * t_fwd, "reverse_ms": 1e3 * t_bwd,
+            "workload": f"{n}x{n} periodic, rho = 1 + 16 Fourier modes, P = 1, v = 0, "
+                        f"B = (0.5, 0.2, 0), dt = {dt}, N = {N}, checkpoint every {every}",
+            "fd_max_rel_density_dirs": rel_dens, "fd_density_dirs": len(dens),
+            "fd_max_rel_smooth_all_fields": rel_sm, "fd_max_rel_smooth_all_fields_eps1e-8": rel_sm8,
+            "fd_smooth_dirs": len(dirs),
+            "fd_smooth_steps": args.adj_smooth_steps, "grad_norm": float(g0.norm().item())}
 
 
 def main():
@@ -152,6 +260,12 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--batch-warmup", type=int, default=3)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--adjoint", type=int, default=1, help="0: skip the adjoint leg")
+    ap.add_argument("--adj-size", type=int, default=512)
+    ap.add_argument("--adj-steps", type=int, default=256)
+    ap.add_argument("--adj-dirs", type=int, default=32)
+    ap.add_argument("--adj-smooth-dirs", type=int, default=8)
+    ap.add_argument("--adj-smooth-steps", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy kernel whose HBM "
                          "counters calibrate FETCH_SIZE/WRITE_SIZE under rocprofv3 --pmc")
@@ -161,6 +275,12 @@ def main():
               "one-GPU build (the multi-GPU halo decomposition is design-only).", file=sys.stderr)
         sys.exit(2)
 
+    if args.adjoint:
+        # torch ships its own HIP runtime under the same soname as the one
+        # libathmi.so links; whichever loads first serves the process, and
+        # torch needs its own
+        import torch  # noqa: F401
+        torch.cuda.init()
     from helpers import load_pkg
     pkg = load_pkg()
     L = pkg.lib()
@@ -178,7 +298,8 @@ def main():
     shape = None
     if not args.no_tune:
         shape = g.autotune(3)
-        log(f"[bench] autotuned shape x2_seg={shape[0]} persist={shape[1]} ({shape[2]:.3f} ms/step)")
+        log(f"[bench] autotuned shape x2_seg={shape[0]} persist={shape[1]} xcd_remap={shape[2]} "
+            f"({shape[3]:.3f} ms/step)")
     for _ in range(args.warmup):
         g.step()
     g.sync()
@@ -197,7 +318,8 @@ def main():
     # kernel-by-kernel pass with HIP events on the handle's stream
     avg = (ctypes.c_double * 16)()
     pkg._check(L.athmi_profile_steps(g._h, args.profile_steps, avg, 16), "profile")
-    kern = {L.athmi_step_kernel_name(k).decode(): avg[k] for k in range(9)}
+    kern = {L.athmi_step_kernel_name(k).decode(): avg[k] for k in range(16)
+            if L.athmi_step_kernel_name(k)}
     dom = max(kern, key=kern.get)
     rd, wr, units = KERNEL_IO[dom]
     alg_bytes = (rd + wr) * 8 * units(n, n)
@@ -215,7 +337,8 @@ def main():
         "config": {"workload": f"MHD blast {n}x{n} (athinput.blast scaled), Roe + PLM, "
                                "CTU+CT, periodic, CFL 0.4; one step = integrate + new_dt + "
                                "set_bvals", "nx1": n, "nx2": n, "variant": args.variant,
-                   "shape": None if shape is None else {"x2_seg": shape[0], "persist": shape[1]}},
+                   "shape": None if shape is None else {"x2_seg": shape[0], "persist": shape[1],
+                                                 "xcd_remap": shape[2]}},
         "roofline": roofline,
         "step_roofline": {"bound": "hbm", "alg_bytes_per_cell_update": ALG_BYTES_PER_CELL,
                           "achieved": step_bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -224,6 +347,8 @@ def main():
     g.close()
     if args.batch > 0:
         out["batched"] = batched_leg(pkg, args, value)
+    if args.adjoint:
+        out["adjoint"] = adjoint_leg(pkg, args)
     if args.calibrate_copy:
         L.athmi_copy_calibration.argtypes = [ctypes.c_size_t, ctypes.c_int]
         pkg._check(L.athmi_copy_calibration(1 << 30, 3), "copy calibration")

@@ -104,16 +104,21 @@ int athmi_sync(athmi_grid *g);
 /* Launch shape: faces marched per x2-predictor thread, persistent
  * workgroups per CU (0 = one tile per workgroup). Picked by the autotuner. */
 int athmi_set_shape(athmi_grid *g, int x2_seg, int persist);
+/* Tile order: 1 = each of the 8 XCDs gets one contiguous run of tiles (its L2
+ * serves the neighbour rows), 0 = launch order. Results are identical. */
+int athmi_set_tile_order(athmi_grid *g, int xcd_remap);
 int athmi_use_graph(athmi_grid *g, int on);
 /* Autotuner: time every launch shape (x2 marching depth 16..128 x persistent
- * workgroups per CU 0..8) over `steps` main-loop steps from the current state
- * through the step graph, restore the state and clocks, and keep the fastest
- * shape (returned with its ms per step). No reference counterpart (the
- * reference launches fixed 32-thread blocks, integrate_2d_cuda.cu:93-193). */
-int athmi_autotune(athmi_grid *g, int steps, int *x2_seg, int *persist, double *ms_per_step);
+ * workgroups per CU 0..8 x tile order) over `steps` main-loop steps from the
+ * current state through the step graph, restore the state and clocks, and
+ * keep the fastest shape (returned with its ms per step). No reference
+ * counterpart (the reference launches fixed 32-thread blocks,
+ * integrate_2d_cuda.cu:93-193). */
+int athmi_autotune(athmi_grid *g, int steps, int *x2_seg, int *persist, int *xcd_remap,
+                   double *ms_per_step);
 /* Time nsteps main-loop steps kernel by kernel (HIP events on the handle's
  * stream between consecutive launches, no graph); avg_ms[k] receives the mean
- * duration of kernel k of the step (nslots >= 9), named by
+ * duration of kernel k of the step (nslots >= 10), named by
  * athmi_step_kernel_name(k). No reference counterpart: measurement hook. */
 int athmi_profile_steps(athmi_grid *g, int nsteps, double *avg_ms, int nslots);
 const char *athmi_step_kernel_name(int k);

@@ -22,7 +22,7 @@ def assert_same(a, b, tag):
         assert np.array_equal(x, y), f"{tag} {name}: max diff {np.abs(x - y).max()}"
 
 
-@pytest.mark.parametrize("shape", [None, (64, 2)])
+@pytest.mark.parametrize("shape", [None, (64, 2, 0), (16, 0, 1)])
 def test_batch_instances_equal_plain_runs(shape):
     pkg, E = ens()
     params = E.ensemble_params(seed=7, n=10)
@@ -30,7 +30,8 @@ def test_batch_instances_equal_plain_runs(shape):
     print(f"[gpu] ensemble seed 7 x10, {steps} steps, shape {shape}", flush=True)
     gb = E.build(params, nlim=steps)
     if shape:
-        gb.set_shape(*shape)
+        gb.set_shape(*shape[:2])
+        gb.set_tile_order(shape[2])
     gb.run()  # issues whole chunks of steps; finished instances idle
     assert all(gb.clock(k)[2] == steps for k in range(gb.n))
     for k, p in enumerate(params):
@@ -72,8 +73,8 @@ def test_autotune_restores_state_and_keeps_results():
     gb = E.build(params, nlim=12)
     before = [gb.download(k) for k in range(gb.n)]
     clocks = [gb.clock(k) for k in range(gb.n)]
-    seg, persist, ms = gb.autotune(1)
-    assert seg in (16, 32, 64, 128) and persist in (0, 1, 2, 4, 8) and ms > 0
+    seg, persist, xcd, ms = gb.autotune(1)
+    assert seg in (16, 32, 64, 128) and persist in (0, 1, 2, 4, 8) and xcd in (0, 1) and ms > 0
     for k in range(gb.n):
         assert_same(gb.download(k), before[k], f"restored {k}")
         assert gb.clock(k) == clocks[k]
