@@ -169,7 +169,7 @@ int athmi_adjoint_backward(athmi_adjoint* h, const double* gradN, double* grad0)
   a.sc = a.g.scratch;
   a.ab = h->ab;
   a.dt = h->dt;
-  a.x2_seg = 32;
+  a.x2_seg = h->g->bl.shape.x2_seg;
   for (int s = h->nck - 1; s >= 0; s--) {
     const int n0 = s * h->every, n1 = std::min(n0 + h->every, h->nsteps);
     // states n0 .. n1-1 of this segment from its checkpoint

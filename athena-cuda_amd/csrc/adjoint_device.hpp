@@ -461,11 +461,13 @@ struct PlmF {
           if (prim_Rnz(m, n)) qab += out_lb[m] * E.R(m, n);
         const double f = c * (E.ev[6] - E.ev[n]);
         const double qa = f * LdW;
-        for (int m = 0; m < 7; m++) Rb[m][n] += out_lb[m] * qa;
-        for (int m = 0; m < 7; m++) {
-          Lb[n][m] += qab * f * dW[m];
-          if (prim_Lnz(n, m)) dWb[m] += qab * f * E.L(n, m);
-        }
+        for (int m = 0; m < 7; m++)
+          if (prim_Rnz(m, n)) Rb[m][n] += out_lb[m] * qa;
+        for (int m = 0; m < 7; m++)
+          if (prim_Lnz(n, m)) {
+            Lb[n][m] += qab * f * dW[m];
+            dWb[m] += qab * f * E.L(n, m);
+          }
         evb[6] += qab * c * LdW;
         evb[n] -= qab * c * LdW;
       } else {
@@ -479,9 +481,11 @@ struct PlmF {
         const double f = c * (E.ev[0] - E.ev[n]);
         const double f2 = c * (E.ev[n] - E.ev[6]);
         for (int m = 0; m < 7; m++) {
-          Rb[m][n] += out_rb[m] * (f * LdW) - out_lb[m] * (f2 * LdW);
-          Lb[n][m] += (qab * f + qa2b * f2) * dW[m];
-          if (prim_Lnz(n, m)) dWb[m] += (qab * f + qa2b * f2) * E.L(n, m);
+          if (prim_Rnz(m, n)) Rb[m][n] += out_rb[m] * (f * LdW) - out_lb[m] * (f2 * LdW);
+          if (prim_Lnz(n, m)) {
+            Lb[n][m] += (qab * f + qa2b * f2) * dW[m];
+            dWb[m] += (qab * f + qa2b * f2) * E.L(n, m);
+          }
         }
         evb[0] += qab * c * LdW;
         evb[n] += -qab * c * LdW + qa2b * c * LdW;
@@ -534,7 +538,8 @@ struct PlmF {
       dab[m] = acc;
     }
     for (int n = 0; n < 7; n++)
-      for (int m = 0; m < 7; m++) Rb[n][m] += dWmb[n] * da[m];
+      for (int m = 0; m < 7; m++)
+        if (prim_Rnz(n, m)) Rb[n][m] += dWmb[n] * da[m];
     // Step 4: limiter
     double dacb[7], dalb[7], darb[7], dagb[7];
     for (int n = 0; n < 7; n++) {
@@ -556,8 +561,8 @@ struct PlmF {
     for (int m = 0; m < 7; m++) {
       double c_ = 0, l_ = 0, r_ = 0, g_ = 0;
       for (int n = 0; n < 7; n++) {
-        Lb[n][m] += dacb[n] * dWc[m] + dalb[n] * dWl[m] + darb[n] * dWr[m] + dagb[n] * dWg[m];
         if (!prim_Lnz(n, m)) continue;
+        Lb[n][m] += dacb[n] * dWc[m] + dalb[n] * dWl[m] + darb[n] * dWr[m] + dagb[n] * dWg[m];
         const double L = E.L(n, m);
         c_ += dacb[n] * L;
         l_ += dalb[n] * L;

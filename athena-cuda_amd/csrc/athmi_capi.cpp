@@ -57,7 +57,11 @@ int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
   g->cpaw_soln.resize(n);
   g->bl.ninst = n;
   g->bl.ncu = prop.multiProcessorCount;
-  g->bl.shape.x2_seg = 32;
+  // faces marched per lane by the x2 predictor: enough 64-lane workgroups
+  // (about 2048, two per SIMD) to fill the chip, 4..16 faces each
+  long cols = 0;
+  for (int k = 0; k < n; k++) cols += (long)cdiv(desc[k].nx1 + 5, kX2Block) * (desc[k].nx2 + 4);
+  g->bl.shape.x2_seg = (int)std::max(4L, std::min(16L, cols / 2048));
   g->bl.shape.persist = 0;
   g->bl.shape.xcd_remap = 1;
   // device memory: per instance 2*NSTATE state planes + NSCRATCH scratch planes

@@ -55,7 +55,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(nx=1024, steps=3):
+def pmc_summary():
+    """Per-kernel HBM traffic (FETCH_SIZE/WRITE_SIZE scaled by the copy-kernel
+    calibration) and FP64 VALU instruction counts from the committed
+    rocprofv3 --pmc summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", "step_kernels_blast4096_pmc.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block)
+    except (OSError, IndexError, ValueError):
+        return None
+
+
+def cpu_baseline(nx=1024, steps=10):
     """The reference's own CPU integrator (oracle/_ref, built from
     /root/reference/src in the build container) on a bounded sample of the
     same workload: blast at nx^2 for a few steps, one core."""
@@ -324,9 +337,14 @@ def main():
     rd, wr, units = KERNEL_IO[dom]
     alg_bytes = (rd + wr) * 8 * units(n, n)
     achieved = alg_bytes / (kern[dom] * 1e-3) / 1e9
+    pmc = pmc_summary() if n == 4096 else None
+    traffic = None
+    if pmc and dom in pmc["kernels"]:
+        traffic = pmc["kernels"][dom]["traffic_bytes"]
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                 "kernel_ms": kern[dom], "alg_bytes_per_launch": alg_bytes,
+                "traffic_source": "profiles/step_kernels_blast4096_pmc.md" if traffic else None,
                 "kernels_ms": kern}
     step_bw = ALG_BYTES_PER_CELL * value / 1e9
     out = {
@@ -343,6 +361,14 @@ def main():
         "step_roofline": {"bound": "hbm", "alg_bytes_per_cell_update": ALG_BYTES_PER_CELL,
                           "achieved": step_bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_bw / HBM_PEAK_GBS},
+        "fp64": None if not pmc else {
+            "flop_per_cell_update": pmc["per_cell_update"]["fp64_flop_fma2"],
+            "lane_instr_per_cell_update": pmc["per_cell_update"]["fp64_lane_instr"],
+            "achieved": pmc["per_cell_update"]["fp64_flop_fma2"] * value / 1e12,
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": pmc["per_cell_update"]["fp64_flop_fma2"] * value / 1e12 / FP64_PEAK_TFLOPS,
+            "hbm_bytes_per_cell_update": pmc["per_cell_update"]["hbm_bytes"],
+            "source": "SQ_INSTS_VALU_*_F64 counts, profiles/step_kernels_blast4096_pmc.md"},
     }
     g.close()
     if args.batch > 0:

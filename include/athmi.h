@@ -102,7 +102,8 @@ int athmi_divb_max(athmi_grid *g, int inst, double *out);
 int athmi_set_stream(athmi_grid *g, void *hip_stream); /* NULL: default stream */
 int athmi_sync(athmi_grid *g);
 /* Launch shape: faces marched per x2-predictor thread, persistent
- * workgroups per CU (0 = one tile per workgroup). Picked by the autotuner. */
+ * workgroups per CU (0 = one tile per workgroup). Picked by the autotuner;
+ * the default marches 4..16 faces so that about 2048 workgroups exist. */
 int athmi_set_shape(athmi_grid *g, int x2_seg, int persist);
 /* Tile order: 1 = each of the 8 XCDs gets one contiguous run of tiles (its L2
  * serves the neighbour rows), 0 = launch order. Results are identical. */

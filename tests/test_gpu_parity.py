@@ -159,3 +159,25 @@ def test_divb_stays_zero_every_step(variant):
             assert dv / scale <= 1e-14, f"{problem} step {s}: max|div B| dx/|B| = {dv / scale}"
             gpu.step()
         gpu.close()
+
+
+@pytest.mark.parametrize("flux", ["roe", "hlle"])
+def test_gpu_cpaw2d_converges_at_second_order(flux):
+    """cpaw2d (the inclined circularly polarised Alfven wave of cpaw2d.c:56-185,
+    an exact travelling solution) run for one wave period on the GPU at
+    N = 64 .. 512 (Lx = 2 Ly): the L1 error against the initial state
+    (cpaw2d.c:222-302) falls at about second order."""
+    pkg = load_pkg()
+    errs = []
+    for n in (64, 128, 256, 512):
+        print(f"[gpu] cpaw2d {flux} {n}x{n} one period", flush=True)
+        g, par, tlim, _ = pkg.setup_problem("cpaw2d", [f"grid/Nx1={n}", f"grid/Nx2={n}"],
+                                            flux=flux)
+        g.set_limits(tlim, -1)
+        g.run()
+        rms = g.cpaw2d_l1(0)[0]
+        errs.append(rms)
+        g.close()
+    orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    print(f"[gpu] cpaw2d {flux} RMS L1 errors {errs} orders {orders}", flush=True)
+    assert np.all(orders > 1.8), orders
