@@ -206,127 +206,178 @@ struct EsysPrimF {
     }
   }
 
-  // Adjoint: given Rb(i,j), Lb(i,j) (adjoints of the matrix entries) and
-  // evb[7], accumulate the adjoints of (d, v1, p, b1, b2, b3) into in_b[6].
-  template <class RB, class LB>
-  ATHMI_HDM void back(const RB& Rb, const LB& Lb, const double* evb, double* in_b) const {
-    double db = 0, v1b = 0, pb = 0, b1b = 0, b2b = 0, b3b = 0;
-    double alpha_fb = 0, alpha_sb = 0, cfb = 0, csb = 0, qsb = 0, qfb = 0, bet2b = 0, bet3b = 0;
-    double asqb = 0, asb = 0, afb = 0, sqrtdb = 0, nab = 0, qsnb = 0, qfnb = 0, aspb = 0;
-    double afpb = 0, dib = 0, vaxb = 0, X;
-    // right eigenvectors
-    X = Rb(0, 0) + Rb(0, 6); db += X * alpha_f; alpha_fb += X * d;
-    X = Rb(0, 2) + Rb(0, 4); db += X * alpha_s; alpha_sb += X * d;
-    X = Rb(1, 6) - Rb(1, 0); cfb += X * alpha_f; alpha_fb += X * cf;
-    X = Rb(1, 4) - Rb(1, 2); csb += X * alpha_s; alpha_sb += X * cs;
-    X = Rb(2, 0) - Rb(2, 6); qsb += X * bet2; bet2b += X * qs;
-    bet3b += Rb(2, 5) - Rb(2, 1);
-    X = Rb(2, 4) - Rb(2, 2); qfb += X * bet2; bet2b += X * qf;
-    X = Rb(3, 0) - Rb(3, 6); qsb += X * bet3; bet3b += X * qs;
-    bet2b += Rb(3, 1) - Rb(3, 5);
-    X = Rb(3, 4) - Rb(3, 2); qfb += X * bet3; bet3b += X * qf;
-    {
-      const double t = d * asq;
-      double tb = 0;
-      X = Rb(4, 0) + Rb(4, 6); alpha_fb += X * t; tb += X * alpha_f;
-      X = Rb(4, 2) + Rb(4, 4); alpha_sb += X * t; tb += X * alpha_s;
-      db += tb * asq; asqb += tb * d;
+  // Adjoints of the intermediates the entries of R and L are built from;
+  // each entry's adjoint is folded in as it arrives (addR, addL), so no
+  // 7x7 adjoint matrix is ever held.
+  struct Acc {
+    double db = 0, alpha_fb = 0, alpha_sb = 0, cfb = 0, csb = 0, qsb = 0, qfb = 0, bet2b = 0;
+    double bet3b = 0, tb = 0, asqb = 0, asb = 0, afb = 0, sqrtdb = 0, nab = 0, qsnb = 0;
+    double qfnb = 0, aspb = 0, afpb = 0, dib = 0;
+  };
+  // A += X * dR(i,j)/d(intermediates); R entries as in R(i, j)
+  ATHMI_HDM void addR(int i, int j, double X, Acc& A) const {
+    switch (i * 7 + j) {
+      case 0: case 6: A.db += X * alpha_f; A.alpha_fb += X * d; break;
+      case 2: case 4: A.db += X * alpha_s; A.alpha_sb += X * d; break;
+      case 7: A.cfb -= X * alpha_f; A.alpha_fb -= X * cf; break;
+      case 13: A.cfb += X * alpha_f; A.alpha_fb += X * cf; break;
+      case 9: A.csb -= X * alpha_s; A.alpha_sb -= X * cs; break;
+      case 11: A.csb += X * alpha_s; A.alpha_sb += X * cs; break;
+      case 14: A.qsb += X * bet2; A.bet2b += X * qs; break;
+      case 20: A.qsb -= X * bet2; A.bet2b -= X * qs; break;
+      case 15: A.bet3b -= X; break;
+      case 19: A.bet3b += X; break;
+      case 16: A.qfb -= X * bet2; A.bet2b -= X * qf; break;
+      case 18: A.qfb += X * bet2; A.bet2b += X * qf; break;
+      case 21: A.qsb += X * bet3; A.bet3b += X * qs; break;
+      case 27: A.qsb -= X * bet3; A.bet3b -= X * qs; break;
+      case 22: A.bet2b += X; break;
+      case 26: A.bet2b -= X; break;
+      case 23: A.qfb -= X * bet3; A.bet3b -= X * qf; break;
+      case 25: A.qfb += X * bet3; A.bet3b += X * qf; break;
+      case 28: case 34: A.alpha_fb += X * d * asq; A.tb += X * alpha_f; break;
+      case 30: case 32: A.alpha_sb += X * d * asq; A.tb += X * alpha_s; break;
+      case 35: case 41: A.asb += X * bet2; A.bet2b += X * as; break;
+      case 36: case 40: A.bet3b -= X * s * sqrtd; A.sqrtdb -= X * bet3 * s; break;
+      case 37: case 39: A.afb -= X * bet2; A.bet2b -= X * af; break;
+      case 42: case 48: A.asb += X * bet3; A.bet3b += X * as; break;
+      case 43: case 47: A.bet2b += X * s * sqrtd; A.sqrtdb += X * bet2 * s; break;
+      case 44: case 46: A.afb -= X * bet3; A.bet3b -= X * af; break;
+      default: break;
     }
-    X = Rb(5, 0) + Rb(5, 6); asb += X * bet2; bet2b += X * as;
-    X = Rb(5, 1) + Rb(5, 5); bet3b -= X * s * sqrtd; sqrtdb -= X * bet3 * s;
-    X = Rb(5, 2) + Rb(5, 4); afb -= X * bet2; bet2b -= X * af;
-    X = Rb(6, 0) + Rb(6, 6); asb += X * bet3; bet3b += X * as;
-    X = Rb(6, 1) + Rb(6, 5); bet2b += X * s * sqrtd; sqrtdb += X * bet2 * s;
-    X = Rb(6, 2) + Rb(6, 4); afb -= X * bet3; bet3b -= X * af;
-    // left eigenvectors
-    X = Lb(0, 1) - Lb(6, 1); nab -= X * cf * alpha_f; cfb -= X * na * alpha_f; alpha_fb -= X * na * cf;
-    X = Lb(0, 2) - Lb(6, 2); qsnb += X * bet2; bet2b += X * qsn;
-    X = Lb(0, 3) - Lb(6, 3); qsnb += X * bet3; bet3b += X * qsn;
-    X = Lb(0, 4) + Lb(6, 4); nab += X * alpha_f * di; alpha_fb += X * na * di; dib += X * na * alpha_f;
-    X = Lb(0, 5) + Lb(6, 5); aspb += X * bet2; bet2b += X * asp;
-    X = Lb(0, 6) + Lb(6, 6); aspb += X * bet3; bet3b += X * asp;
-    X = Lb(1, 2) - Lb(5, 2); bet3b -= 0.5 * X;
-    X = Lb(1, 3) - Lb(5, 3); bet2b += 0.5 * X;
-    X = Lb(1, 5) + Lb(5, 5); bet3b -= 0.5 * s / sqrtd * X; sqrtdb += X * 0.5 * bet3 * s / (sqrtd * sqrtd);
-    X = Lb(1, 6) + Lb(5, 6); bet2b += 0.5 * s / sqrtd * X; sqrtdb -= X * 0.5 * bet2 * s / (sqrtd * sqrtd);
-    X = Lb(2, 1) - Lb(4, 1); nab -= X * cs * alpha_s; csb -= X * na * alpha_s; alpha_sb -= X * na * cs;
-    X = Lb(2, 2) - Lb(4, 2); qfnb -= X * bet2; bet2b -= X * qfn;
-    X = Lb(2, 3) - Lb(4, 3); qfnb -= X * bet3; bet3b -= X * qfn;
-    X = Lb(2, 4) + Lb(4, 4); nab += X * alpha_s * di; alpha_sb += X * na * di; dib += X * na * alpha_s;
-    X = Lb(2, 5) + Lb(4, 5); afpb -= X * bet2; bet2b -= X * afp;
-    X = Lb(2, 6) + Lb(4, 6); afpb -= X * bet3; bet3b -= X * afp;
-    asqb += Lb(3, 4) / (asq * asq);
+  }
+  // A += X * dL(i,j)/d(intermediates); L entries as in L(i, j)
+  ATHMI_HDM void addL(int i, int j, double X, Acc& A) const {
+    switch (i * 7 + j) {
+      case 1: case 43: {
+        const double Y = (i == 0) ? X : -X;  // L61 = -L01 = na cf alpha_f
+        A.nab -= Y * cf * alpha_f; A.cfb -= Y * na * alpha_f; A.alpha_fb -= Y * na * cf;
+        break;
+      }
+      case 2: A.qsnb += X * bet2; A.bet2b += X * qsn; break;
+      case 44: A.qsnb -= X * bet2; A.bet2b -= X * qsn; break;
+      case 3: A.qsnb += X * bet3; A.bet3b += X * qsn; break;
+      case 45: A.qsnb -= X * bet3; A.bet3b -= X * qsn; break;
+      case 4: case 46: A.nab += X * alpha_f * di; A.alpha_fb += X * na * di; A.dib += X * na * alpha_f; break;
+      case 5: case 47: A.aspb += X * bet2; A.bet2b += X * asp; break;
+      case 6: case 48: A.aspb += X * bet3; A.bet3b += X * asp; break;
+      case 9: A.bet3b -= 0.5 * X; break;
+      case 37: A.bet3b += 0.5 * X; break;
+      case 10: A.bet2b += 0.5 * X; break;
+      case 38: A.bet2b -= 0.5 * X; break;
+      case 12: case 40:
+        A.bet3b -= 0.5 * s / sqrtd * X; A.sqrtdb += X * 0.5 * bet3 * s / (sqrtd * sqrtd); break;
+      case 13: case 41:
+        A.bet2b += 0.5 * s / sqrtd * X; A.sqrtdb -= X * 0.5 * bet2 * s / (sqrtd * sqrtd); break;
+      case 15: case 29: {
+        const double Y = (i == 2) ? X : -X;  // L41 = -L21
+        A.nab -= Y * cs * alpha_s; A.csb -= Y * na * alpha_s; A.alpha_sb -= Y * na * cs;
+        break;
+      }
+      case 16: A.qfnb -= X * bet2; A.bet2b -= X * qfn; break;
+      case 30: A.qfnb += X * bet2; A.bet2b += X * qfn; break;
+      case 17: A.qfnb -= X * bet3; A.bet3b -= X * qfn; break;
+      case 31: A.qfnb += X * bet3; A.bet3b += X * qfn; break;
+      case 18: case 32: A.nab += X * alpha_s * di; A.alpha_sb += X * na * di; A.dib += X * na * alpha_s; break;
+      case 19: case 33: A.afpb -= X * bet2; A.bet2b -= X * afp; break;
+      case 20: case 34: A.afpb -= X * bet3; A.bet3b -= X * afp; break;
+      case 25: A.asqb += X / (asq * asq); break;
+      default: break;
+    }
+  }
+
+  // Adjoint from the folded entry adjoints A and evb[7]: accumulates the
+  // adjoints of (d, v1, p, b1, b2, b3) into in_b[6].
+  ATHMI_HDM void back_acc(const Acc& A0, const double* evb, double* in_b) const {
+    Acc A = A0;
+    double v1b = 0, pb = 0, b1b = 0, b2b = 0, b3b = 0, vaxb = 0;
+    A.db += A.tb * asq;
+    A.asqb += A.tb * d;
     // L prefixes: afp = na*af*di, asp = na*as*di, qfn = na*qf, qsn = na*qs, na = 0.5/asq
-    nab += afpb * af * di + aspb * as * di + qfnb * qf + qsnb * qs;
-    afb += afpb * na * di;
-    asb += aspb * na * di;
-    dib += afpb * na * af + aspb * na * as;
-    qfb += qfnb * na;
-    qsb += qsnb * na;
-    asqb -= nab * na / asq;
+    A.nab += A.afpb * af * di + A.aspb * as * di + A.qfnb * qf + A.qsnb * qs;
+    A.afb += A.afpb * na * di;
+    A.asb += A.aspb * na * di;
+    A.dib += A.afpb * na * af + A.aspb * na * as;
+    A.qfb += A.qfnb * na;
+    A.qsb += A.qsnb * na;
+    A.asqb -= A.nab * na / asq;
     // eigenvalues
     for (int k = 0; k < 7; k++) v1b += evb[k];
-    cfb += evb[6] - evb[0];
+    A.cfb += evb[6] - evb[0];
     vaxb += evb[5] - evb[1];
-    csb += evb[4] - evb[2];
+    A.csb += evb[4] - evb[2];
     double vaxsqb = sqrt_b(vax, vaxb);
     // af = a*alpha_f*sqrtd, as = a*alpha_s*sqrtd
-    double ab = afb * alpha_f * sqrtd + asb * alpha_s * sqrtd;
-    alpha_fb += afb * a * sqrtd;
-    alpha_sb += asb * a * sqrtd;
-    sqrtdb += afb * a * alpha_f + asb * a * alpha_s;
+    const double ab = A.afb * alpha_f * sqrtd + A.asb * alpha_s * sqrtd;
+    A.alpha_fb += A.afb * a * sqrtd;
+    A.alpha_sb += A.asb * a * sqrtd;
+    A.sqrtdb += A.afb * a * alpha_f + A.asb * a * alpha_s;
     // qf = cf*alpha_f*s, qs = cs*alpha_s*s
-    cfb += qfb * alpha_f * s;
-    alpha_fb += qfb * cf * s;
-    csb += qsb * alpha_s * s;
-    alpha_sb += qsb * cs * s;
-    asqb += sqrt_b(a, ab);
-    db += sqrt_b(sqrtd, sqrtdb);
+    A.cfb += A.qfb * alpha_f * s;
+    A.alpha_fb += A.qfb * cf * s;
+    A.csb += A.qsb * alpha_s * s;
+    A.alpha_sb += A.qsb * cs * s;
+    A.asqb += sqrt_b(a, ab);
+    A.db += sqrt_b(sqrtd, A.sqrtdb);
     double cssqb = 0, cfsqb = 0, cf2_cs2b = 0;
     if (abr) {
       const double Qf = (asq - cssq) / cf2_cs2, Qs = (cfsq - asq) / cf2_cs2;
-      const double Qfb = sqrt_b(alpha_f, alpha_fb), Qsb = sqrt_b(alpha_s, alpha_sb);
-      asqb += (Qfb - Qsb) / cf2_cs2;
+      const double Qfb = sqrt_b(alpha_f, A.alpha_fb), Qsb = sqrt_b(alpha_s, A.alpha_sb);
+      A.asqb += (Qfb - Qsb) / cf2_cs2;
       cssqb -= Qfb / cf2_cs2;
       cfsqb += Qsb / cf2_cs2;
       cf2_cs2b -= (Qfb * Qf + Qsb * Qs) / cf2_cs2;
     }
     double btsqb = 0;
     if (!btz) {
-      b2b += bet2b / bt;
-      b3b += bet3b / bt;
-      const double btb = -(bet2b * bet2 + bet3b * bet3) / bt;
+      b2b += A.bet2b / bt;
+      b3b += A.bet3b / bt;
+      const double btb = -(A.bet2b * bet2 + A.bet3b * bet3) / bt;
       btsqb += sqrt_b(bt, btb);
     }
-    cssqb += sqrt_b(cs, csb);
-    asqb += cssqb * vaxsq / cfsq;
+    cssqb += sqrt_b(cs, A.csb);
+    A.asqb += cssqb * vaxsq / cfsq;
     vaxsqb += cssqb * asq / cfsq;
     cfsqb -= cssqb * cssq / cfsq;
-    cfsqb += sqrt_b(cf, cfb);
+    cfsqb += sqrt_b(cf, A.cfb);
     const double tsumb = 0.5 * cfsqb;
     cf2_cs2b += 0.5 * cfsqb;
     const double Yb = sqrt_b(cf2_cs2, cf2_cs2b);
     const double tdifb = 2.0 * tdif * Yb;
-    asqb += 4.0 * ct2 * Yb;
+    A.asqb += 4.0 * ct2 * Yb;
     double ct2b = 4.0 * asq * Yb;
     vaxsqb += tsumb + tdifb;
     ct2b += tsumb + tdifb;
-    asqb += tsumb - tdifb;
+    A.asqb += tsumb - tdifb;
     btsqb += ct2b * di;
-    dib += ct2b * btsq;
-    pb += asqb * gamma * di;
-    dib += asqb * gamma * p;
+    A.dib += ct2b * btsq;
+    pb += A.asqb * gamma * di;
+    A.dib += A.asqb * gamma * p;
     b1b += vaxsqb * 2.0 * b1 * di;
-    dib += vaxsqb * b1 * b1;
+    A.dib += vaxsqb * b1 * b1;
     b2b += 2.0 * b2 * btsqb;
     b3b += 2.0 * b3 * btsqb;
-    db -= dib * di * di;
-    in_b[0] += db;
+    A.db -= A.dib * di * di;
+    in_b[0] += A.db;
     in_b[1] += v1b;
     in_b[2] += pb;
     in_b[3] += b1b;
     in_b[4] += b2b;
     in_b[5] += b3b;
+  }
+
+  // Adjoint from matrices of entry adjoints Rb(i,j), Lb(i,j) and evb[7]
+  template <class RB, class LB>
+  ATHMI_HDM void back(const RB& Rb, const LB& Lb, const double* evb, double* in_b) const {
+    Acc A;
+#pragma unroll
+    for (int i = 0; i < 7; i++)
+#pragma unroll
+      for (int j = 0; j < 7; j++) {
+        addR(i, j, Rb(i, j), A);
+        addL(i, j, Lb(i, j), A);
+      }
+    back_acc(A, evb, in_b);
   }
 };
 
@@ -436,43 +487,47 @@ struct PlmF {
     }
   }
 
-  // Adjoint: out_lb, out_rb -> (+=) wmb, w0b, wpb, bxcb
+  // Adjoint: out_lb, out_rb -> (+=) wmb, w0b, wpb, bxcb. The adjoints of the
+  // eigenmatrix entries are folded into the eigensystem's intermediates as
+  // they arise (EsysPrimF::addR / addL).
   ATHMI_HDM void back(const double* out_lb, const double* out_rb, double* wmb, double* w0b,
-                     double* wpb, double& bxcb) const {
-    double Rb[7][7], Lb[7][7], evb[7], dWb[7];
+                      double* wpb, double& bxcb) const {
+    EsysPrimF::Acc A;
+    double evb[7], dWb[7];
     for (int n = 0; n < 7; n++) {
       evb[n] = 0.0;
       dWb[n] = 0.0;
-      for (int m = 0; m < 7; m++) {
-        Rb[n][m] = 0.0;
-        Lb[n][m] = 0.0;
-      }
     }
     const double c = 0.5 * dtodx;
     // Step 9
+#pragma unroll
     for (int n = 0; n < 7; n++) {
       if (!(E.ev[n] > 0.) && !(E.ev[n] < 0.)) continue;
       double LdW = 0.0;
+#pragma unroll
       for (int m = 0; m < 7; m++)
         if (prim_Lnz(n, m)) LdW += E.L(n, m) * dW[m];
       if (E.ev[n] > 0.) {  // out_l += qa R[:,n], qa = c (ev6 - evn) LdW
         double qab = 0.0;
+#pragma unroll
         for (int m = 0; m < 7; m++)
           if (prim_Rnz(m, n)) qab += out_lb[m] * E.R(m, n);
         const double f = c * (E.ev[6] - E.ev[n]);
         const double qa = f * LdW;
-        for (int m = 0; m < 7; m++)
-          if (prim_Rnz(m, n)) Rb[m][n] += out_lb[m] * qa;
-        for (int m = 0; m < 7; m++)
+#pragma unroll
+        for (int m = 0; m < 7; m++) {
+          if (prim_Rnz(m, n)) E.addR(m, n, out_lb[m] * qa, A);
           if (prim_Lnz(n, m)) {
-            Lb[n][m] += qab * f * dW[m];
+            E.addL(n, m, qab * f * dW[m], A);
             dWb[m] += qab * f * E.L(n, m);
           }
+        }
         evb[6] += qab * c * LdW;
         evb[n] -= qab * c * LdW;
       } else {
-        // out_r += qa R[:,n], qa = c (ev0 - evn) LdW
+        // out_r += qa R[:,n], qa = c (ev0 - evn) LdW; out_l -= qa2 R[:,n]
         double qab = 0.0, qa2b = 0.0;
+#pragma unroll
         for (int m = 0; m < 7; m++)
           if (prim_Rnz(m, n)) {
             qab += out_rb[m] * E.R(m, n);
@@ -480,11 +535,13 @@ struct PlmF {
           }
         const double f = c * (E.ev[0] - E.ev[n]);
         const double f2 = c * (E.ev[n] - E.ev[6]);
+        const double k = qab * f + qa2b * f2;
+#pragma unroll
         for (int m = 0; m < 7; m++) {
-          if (prim_Rnz(m, n)) Rb[m][n] += out_rb[m] * (f * LdW) - out_lb[m] * (f2 * LdW);
+          if (prim_Rnz(m, n)) E.addR(m, n, out_rb[m] * (f * LdW) - out_lb[m] * (f2 * LdW), A);
           if (prim_Lnz(n, m)) {
-            Lb[n][m] += (qab * f + qa2b * f2) * dW[m];
-            dWb[m] += (qab * f + qa2b * f2) * E.L(n, m);
+            E.addL(n, m, k * dW[m], A);
+            dWb[m] += k * E.L(n, m);
           }
         }
         evb[0] += qab * c * LdW;
@@ -494,6 +551,7 @@ struct PlmF {
     }
     // Step 8
     double Wlvb[7], Wrvb[7], qx1b = 0.0, qx2b = 0.0;
+#pragma unroll
     for (int m = 0; m < 7; m++) {
       Wrvb[m] = out_lb[m];
       qx1b -= out_lb[m] * dW[m];
@@ -504,6 +562,7 @@ struct PlmF {
     if (E.ev[6] > 0.0) evb[6] += qx1b * 0.5 * dtodx;
     if (E.ev[0] < 0.0) evb[0] -= qx2b * 0.5 * dtodx;
     double dWmb[7];
+#pragma unroll
     for (int n = 0; n < 7; n++) {
       Wrvb[n] += dWb[n];
       Wlvb[n] -= dWb[n];
@@ -531,17 +590,20 @@ struct PlmF {
     }
     // Step 5: dWm = R da
     double dab[7];
+#pragma unroll
     for (int m = 0; m < 7; m++) {
       double acc = 0.0;
+#pragma unroll
       for (int n = 0; n < 7; n++)
-        if (prim_Rnz(n, m)) acc += dWmb[n] * E.R(n, m);
+        if (prim_Rnz(n, m)) {
+          acc += dWmb[n] * E.R(n, m);
+          E.addR(n, m, dWmb[n] * da[m], A);
+        }
       dab[m] = acc;
     }
-    for (int n = 0; n < 7; n++)
-      for (int m = 0; m < 7; m++)
-        if (prim_Rnz(n, m)) Rb[n][m] += dWmb[n] * da[m];
     // Step 4: limiter
     double dacb[7], dalb[7], darb[7], dagb[7];
+#pragma unroll
     for (int n = 0; n < 7; n++) {
       dacb[n] = dalb[n] = darb[n] = dagb[n] = 0.0;
       if (dal[n] * dar[n] > 0.0) {
@@ -556,39 +618,33 @@ struct PlmF {
         else dagb[n] += ls2b * sgnabs(dag[n]);
       }
     }
-    // Step 3: projections with L
-    double dWcb[7], dWlb[7], dWrb[7], dWgb[7];
+    // Step 3: projections with L, then Step 2
+#pragma unroll
     for (int m = 0; m < 7; m++) {
       double c_ = 0, l_ = 0, r_ = 0, g_ = 0;
+#pragma unroll
       for (int n = 0; n < 7; n++) {
         if (!prim_Lnz(n, m)) continue;
-        Lb[n][m] += dacb[n] * dWc[m] + dalb[n] * dWl[m] + darb[n] * dWr[m] + dagb[n] * dWg[m];
+        E.addL(n, m, dacb[n] * dWc[m] + dalb[n] * dWl[m] + darb[n] * dWr[m] + dagb[n] * dWg[m], A);
         const double L = E.L(n, m);
         c_ += dacb[n] * L;
         l_ += dalb[n] * L;
         r_ += darb[n] * L;
         g_ += dagb[n] * L;
       }
-      dWcb[m] = c_;
-      dWlb[m] = l_;
-      dWrb[m] = r_;
-      dWgb[m] = g_;
-    }
-    // Step 2
-    for (int n = 0; n < 7; n++) {
-      if (dWl[n] * dWr[n] > 0.0) {
-        const double S = dWl[n] + dWr[n];
-        dWlb[n] += dWgb[n] * 2.0 * dWr[n] * dWr[n] / (S * S);
-        dWrb[n] += dWgb[n] * 2.0 * dWl[n] * dWl[n] / (S * S);
+      double dWlb = l_, dWrb = r_;
+      if (dWl[m] * dWr[m] > 0.0) {
+        const double S = dWl[m] + dWr[m];
+        dWlb += g_ * 2.0 * dWr[m] * dWr[m] / (S * S);
+        dWrb += g_ * 2.0 * dWl[m] * dWl[m] / (S * S);
       }
-      wpb[n] += dWcb[n] + dWrb[n];
-      wmb[n] -= dWcb[n] + dWlb[n];
-      w0b[n] += dWlb[n] - dWrb[n];
+      wpb[m] += c_ + dWrb;
+      wmb[m] -= c_ + dWlb;
+      w0b[m] += dWlb - dWrb;
     }
     // eigensystem of W[i] (d, vx, p, Bxc, by, bz)
     double inb[6] = {0, 0, 0, 0, 0, 0};
-    E.back([&](int i, int j) { return Rb[i][j]; }, [&](int i, int j) { return Lb[i][j]; }, evb,
-           inb);
+    E.back_acc(A, evb, inb);
     w0b[0] += inb[0];
     w0b[1] += inb[1];
     w0b[4] += inb[2];
