@@ -396,17 +396,77 @@ ATHMI_HD constexpr bool prim_Rnz(int n, int m) {
 
 // ---------------------------------------------------------------------------
 // lr_states_plm.c:64-248 for one cell i (W[i-1], W[i], W[i+1], Bxc[i]):
-// out_l -> Wl[i+1], out_r -> Wr[i]. Forward record in the reference order.
+// out_l -> Wl[i+1], out_r -> Wr[i]. Only the stencil and the eigensystem are
+// kept; the forward chain (Steps 2-7) is recomputed, in the reference order,
+// by each reverse stage that needs a part of it, which keeps few values live.
 struct PlmF {
   double wm[7], w0[7], wp[7], bxc, dtodx;
   EsysPrimF E;
-  double dWc[7], dWl[7], dWr[7], dWg[7], dac[7], dal[7], dar[7], dag[7], da[7], dWm[7];
-  double Wlv0[7], Wrv0[7], dW[7];
-  double qx1, qx2;
   double out_l[7], out_r[7];
 
+  // Steps 2-4 for component n: the four characteristic projections and the
+  // limited slope
+  ATHMI_HDM void diffs(int m, double& c, double& l, double& r, double& g) const {
+    c = wp[m] - wm[m];
+    l = w0[m] - wm[m];
+    r = wp[m] - w0[m];
+    g = (l * r > 0.0) ? 2.0 * l * r / (l + r) : 0.0;
+  }
+  ATHMI_HDM void project(int n, double& ac, double& al, double& ar, double& ag) const {
+    double c = 0.0, l = 0.0, r = 0.0, g = 0.0;
+#pragma unroll
+    for (int m = 0; m < 7; m++) {
+      if (!prim_Lnz(n, m)) continue;
+      double dc, dl, dr, dg;
+      diffs(m, dc, dl, dr, dg);
+      const double L = E.L(n, m);
+      c = c + L * dc;
+      l = l + L * dl;
+      r = r + L * dr;
+      g = g + L * dg;
+    }
+    ac = c;
+    al = l;
+    ar = r;
+    ag = g;
+  }
+  ATHMI_HDM double limited(double ac, double al, double ar, double ag) const {
+    if (!(al * ar > 0.0)) return 0.0;
+    const double ls1 = dmin(fabs(al), fabs(ar));
+    const double ls2 = dmin(0.5 * fabs(ac), fabs(ag));
+    return dsign(ac) * dmin(2.0 * ls1, ls2);
+  }
+  // Steps 2-7: slopes da, the unclamped Wlv0/Wrv0, the clamped Wlv/Wrv and
+  // dW = Wrv - Wlv (any output may be null)
+  ATHMI_HDM void chain(double* da_out, double* Wlv0, double* Wrv0, double* Wlv, double* Wrv,
+                       double* dW) const {
+    double da[7];
+#pragma unroll
+    for (int n = 0; n < 7; n++) {
+      double ac, al, ar, ag;
+      project(n, ac, al, ar, ag);
+      da[n] = limited(ac, al, ar, ag);
+      if (da_out) da_out[n] = da[n];
+    }
+#pragma unroll
+    for (int n = 0; n < 7; n++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int m = 0; m < 7; m++)
+        if (prim_Rnz(n, m)) acc = acc + da[m] * E.R(n, m);
+      const double l0 = w0[n] - 0.5 * acc, r0 = w0[n] + 0.5 * acc;
+      const double l = dmin(dmax(w0[n], wm[n]), dmax(dmin(w0[n], wm[n]), l0));
+      const double r = dmin(dmax(w0[n], wp[n]), dmax(dmin(w0[n], wp[n]), r0));
+      if (Wlv0) Wlv0[n] = l0;
+      if (Wrv0) Wrv0[n] = r0;
+      if (Wlv) Wlv[n] = l;
+      if (Wrv) Wrv[n] = r;
+      if (dW) dW[n] = r - l;
+    }
+  }
+
   ATHMI_HDM void run(const Prim7& Wm, const Prim7& W0, const Prim7& Wp, double bxc_,
-                    double dtodx_, const EosC& eos) {
+                     double dtodx_, const EosC& eos) {
     for (int n = 0; n < 7; n++) {
       wm[n] = arr(Wm)[n];
       w0[n] = arr(W0)[n];
@@ -415,52 +475,11 @@ struct PlmF {
     bxc = bxc_;
     dtodx = dtodx_;
     E.run(W0.d, W0.vx, W0.p, bxc, W0.by, W0.bz, eos.gamma);
-    for (int n = 0; n < 7; n++) {  // Step 2 (:108-117)
-      dWc[n] = wp[n] - wm[n];
-      dWl[n] = w0[n] - wm[n];
-      dWr[n] = wp[n] - w0[n];
-      dWg[n] = (dWl[n] * dWr[n] > 0.0) ? 2.0 * dWl[n] * dWr[n] / (dWl[n] + dWr[n]) : 0.0;
-    }
-    for (int n = 0; n < 7; n++) {  // Step 3 (:122-133)
-      double c = 0.0, l = 0.0, r = 0.0, g = 0.0;
-      for (int m = 0; m < 7; m++) {
-        if (!prim_Lnz(n, m)) continue;
-        const double L = E.L(n, m);
-        c = c + L * dWc[m];
-        l = l + L * dWl[m];
-        r = r + L * dWr[m];
-        g = g + L * dWg[m];
-      }
-      dac[n] = c;
-      dal[n] = l;
-      dar[n] = r;
-      dag[n] = g;
-    }
-    for (int n = 0; n < 7; n++) {  // Step 4 (:138-145)
-      da[n] = 0.0;
-      if (dal[n] * dar[n] > 0.0) {
-        const double ls1 = dmin(fabs(dal[n]), fabs(dar[n]));
-        const double ls2 = dmin(0.5 * fabs(dac[n]), fabs(dag[n]));
-        da[n] = dsign(dac[n]) * dmin(2.0 * ls1, ls2);
-      }
-    }
-    for (int n = 0; n < 7; n++) {  // Step 5 (:150-155)
-      double acc = 0.0;
-      for (int m = 0; m < 7; m++)
-        if (prim_Rnz(n, m)) acc = acc + da[m] * E.R(n, m);
-      dWm[n] = acc;
-    }
-    double Wlv[7], Wrv[7];
-    for (int n = 0; n < 7; n++) {  // Step 7 (:160-174)
-      Wlv0[n] = w0[n] - 0.5 * dWm[n];
-      Wrv0[n] = w0[n] + 0.5 * dWm[n];
-      Wlv[n] = dmin(dmax(w0[n], wm[n]), dmax(dmin(w0[n], wm[n]), Wlv0[n]));
-      Wrv[n] = dmin(dmax(w0[n], wp[n]), dmax(dmin(w0[n], wp[n]), Wrv0[n]));
-      dW[n] = Wrv[n] - Wlv[n];
-    }
-    qx1 = 0.5 * dmax(E.ev[6], 0.0) * dtodx;  // Step 8 (:181-192)
+    double Wlv[7], Wrv[7], dW[7];
+    chain(nullptr, nullptr, nullptr, Wlv, Wrv, dW);
+    const double qx1 = 0.5 * dmax(E.ev[6], 0.0) * dtodx;  // Step 8 (:181-192)
     for (int n = 0; n < 7; n++) out_l[n] = Wrv[n] - qx1 * dW[n];
-    qx2 = -0.5 * dmin(E.ev[0], 0.0) * dtodx;
+    const double qx2 = -0.5 * dmin(E.ev[0], 0.0) * dtodx;
     for (int n = 0; n < 7; n++) out_r[n] = Wlv[n] + qx2 * dW[n];
     for (int n = 0; n < 7; n++) {  // Step 9 (:201-209)
       if (E.ev[n] > 0.) {
@@ -499,148 +518,160 @@ struct PlmF {
       dWb[n] = 0.0;
     }
     const double c = 0.5 * dtodx;
-    // Step 9
-#pragma unroll
-    for (int n = 0; n < 7; n++) {
-      if (!(E.ev[n] > 0.) && !(E.ev[n] < 0.)) continue;
-      double LdW = 0.0;
-#pragma unroll
-      for (int m = 0; m < 7; m++)
-        if (prim_Lnz(n, m)) LdW += E.L(n, m) * dW[m];
-      if (E.ev[n] > 0.) {  // out_l += qa R[:,n], qa = c (ev6 - evn) LdW
-        double qab = 0.0;
-#pragma unroll
-        for (int m = 0; m < 7; m++)
-          if (prim_Rnz(m, n)) qab += out_lb[m] * E.R(m, n);
-        const double f = c * (E.ev[6] - E.ev[n]);
-        const double qa = f * LdW;
-#pragma unroll
-        for (int m = 0; m < 7; m++) {
-          if (prim_Rnz(m, n)) E.addR(m, n, out_lb[m] * qa, A);
-          if (prim_Lnz(n, m)) {
-            E.addL(n, m, qab * f * dW[m], A);
-            dWb[m] += qab * f * E.L(n, m);
-          }
-        }
-        evb[6] += qab * c * LdW;
-        evb[n] -= qab * c * LdW;
-      } else {
-        // out_r += qa R[:,n], qa = c (ev0 - evn) LdW; out_l -= qa2 R[:,n]
-        double qab = 0.0, qa2b = 0.0;
-#pragma unroll
-        for (int m = 0; m < 7; m++)
-          if (prim_Rnz(m, n)) {
-            qab += out_rb[m] * E.R(m, n);
-            qa2b -= out_lb[m] * E.R(m, n);
-          }
-        const double f = c * (E.ev[0] - E.ev[n]);
-        const double f2 = c * (E.ev[n] - E.ev[6]);
-        const double k = qab * f + qa2b * f2;
-#pragma unroll
-        for (int m = 0; m < 7; m++) {
-          if (prim_Rnz(m, n)) E.addR(m, n, out_rb[m] * (f * LdW) - out_lb[m] * (f2 * LdW), A);
-          if (prim_Lnz(n, m)) {
-            E.addL(n, m, k * dW[m], A);
-            dWb[m] += k * E.L(n, m);
-          }
-        }
-        evb[0] += qab * c * LdW;
-        evb[n] += -qab * c * LdW + qa2b * c * LdW;
-        evb[6] -= qa2b * c * LdW;
-      }
-    }
-    // Step 8
-    double Wlvb[7], Wrvb[7], qx1b = 0.0, qx2b = 0.0;
-#pragma unroll
-    for (int m = 0; m < 7; m++) {
-      Wrvb[m] = out_lb[m];
-      qx1b -= out_lb[m] * dW[m];
-      Wlvb[m] = out_rb[m];
-      qx2b += out_rb[m] * dW[m];
-      dWb[m] += qx2 * out_rb[m] - qx1 * out_lb[m];
-    }
-    if (E.ev[6] > 0.0) evb[6] += qx1b * 0.5 * dtodx;
-    if (E.ev[0] < 0.0) evb[0] -= qx2b * 0.5 * dtodx;
-    double dWmb[7];
-#pragma unroll
-    for (int n = 0; n < 7; n++) {
-      Wrvb[n] += dWb[n];
-      Wlvb[n] -= dWb[n];
-      // Wlv = MIN(MAX(w0,wm), MAX(MIN(w0,wm), Wlv0))
-      double lob = 0, hib = 0, s1b = 0, Wlv0b = 0, Wrv0b = 0;
-      {
-        const double lo = dmin(w0[n], wm[n]), hi = dmax(w0[n], wm[n]);
-        const double s1 = dmax(lo, Wlv0[n]);
-        if (hi < s1) hib = Wlvb[n]; else s1b = Wlvb[n];
-        if (lo > Wlv0[n]) lob = s1b; else Wlv0b = s1b;
-        if (w0[n] < wm[n]) w0b[n] += lob; else wmb[n] += lob;
-        if (w0[n] > wm[n]) w0b[n] += hib; else wmb[n] += hib;
-      }
-      {
-        lob = hib = s1b = 0;
-        const double lo = dmin(w0[n], wp[n]), hi = dmax(w0[n], wp[n]);
-        const double s1 = dmax(lo, Wrv0[n]);
-        if (hi < s1) hib = Wrvb[n]; else s1b = Wrvb[n];
-        if (lo > Wrv0[n]) lob = s1b; else Wrv0b = s1b;
-        if (w0[n] < wp[n]) w0b[n] += lob; else wpb[n] += lob;
-        if (w0[n] > wp[n]) w0b[n] += hib; else wpb[n] += hib;
-      }
-      w0b[n] += Wlv0b + Wrv0b;
-      dWmb[n] = 0.5 * (Wrv0b - Wlv0b);
-    }
-    // Step 5: dWm = R da
-    double dab[7];
-#pragma unroll
-    for (int m = 0; m < 7; m++) {
-      double acc = 0.0;
-#pragma unroll
-      for (int n = 0; n < 7; n++)
-        if (prim_Rnz(n, m)) {
-          acc += dWmb[n] * E.R(n, m);
-          E.addR(n, m, dWmb[n] * da[m], A);
-        }
-      dab[m] = acc;
-    }
-    // Step 4: limiter
-    double dacb[7], dalb[7], darb[7], dagb[7];
-#pragma unroll
-    for (int n = 0; n < 7; n++) {
-      dacb[n] = dalb[n] = darb[n] = dagb[n] = 0.0;
-      if (dal[n] * dar[n] > 0.0) {
-        const double ls1 = dmin(fabs(dal[n]), fabs(dar[n]));
-        const double ls2 = dmin(0.5 * fabs(dac[n]), fabs(dag[n]));
-        const double sg = dsign(dac[n]);
-        double ls1b = 0.0, ls2b = 0.0;
-        if (2.0 * ls1 < ls2) ls1b = 2.0 * sg * dab[n]; else ls2b = sg * dab[n];
-        if (fabs(dal[n]) < fabs(dar[n])) dalb[n] += ls1b * sgnabs(dal[n]);
-        else darb[n] += ls1b * sgnabs(dar[n]);
-        if (0.5 * fabs(dac[n]) < fabs(dag[n])) dacb[n] += ls2b * 0.5 * sgnabs(dac[n]);
-        else dagb[n] += ls2b * sgnabs(dag[n]);
-      }
-    }
-    // Step 3: projections with L, then Step 2
-#pragma unroll
-    for (int m = 0; m < 7; m++) {
-      double c_ = 0, l_ = 0, r_ = 0, g_ = 0;
+    {
+      double dW[7];
+      chain(nullptr, nullptr, nullptr, nullptr, nullptr, dW);
+      // Step 9
 #pragma unroll
       for (int n = 0; n < 7; n++) {
+        if (!(E.ev[n] > 0.) && !(E.ev[n] < 0.)) continue;
+        double LdW = 0.0;
+#pragma unroll
+        for (int m = 0; m < 7; m++)
+          if (prim_Lnz(n, m)) LdW += E.L(n, m) * dW[m];
+        if (E.ev[n] > 0.) {  // out_l += qa R[:,n], qa = c (ev6 - evn) LdW
+          double qab = 0.0;
+#pragma unroll
+          for (int m = 0; m < 7; m++)
+            if (prim_Rnz(m, n)) qab += out_lb[m] * E.R(m, n);
+          const double f = c * (E.ev[6] - E.ev[n]);
+          const double qa = f * LdW;
+#pragma unroll
+          for (int m = 0; m < 7; m++) {
+            if (prim_Rnz(m, n)) E.addR(m, n, out_lb[m] * qa, A);
+            if (prim_Lnz(n, m)) {
+              E.addL(n, m, qab * f * dW[m], A);
+              dWb[m] += qab * f * E.L(n, m);
+            }
+          }
+          evb[6] += qab * c * LdW;
+          evb[n] -= qab * c * LdW;
+        } else {
+          // out_r += qa R[:,n], qa = c (ev0 - evn) LdW; out_l -= qa2 R[:,n]
+          double qab = 0.0, qa2b = 0.0;
+#pragma unroll
+          for (int m = 0; m < 7; m++)
+            if (prim_Rnz(m, n)) {
+              qab += out_rb[m] * E.R(m, n);
+              qa2b -= out_lb[m] * E.R(m, n);
+            }
+          const double f = c * (E.ev[0] - E.ev[n]);
+          const double f2 = c * (E.ev[n] - E.ev[6]);
+          const double k = qab * f + qa2b * f2;
+#pragma unroll
+          for (int m = 0; m < 7; m++) {
+            if (prim_Rnz(m, n)) E.addR(m, n, out_rb[m] * (f * LdW) - out_lb[m] * (f2 * LdW), A);
+            if (prim_Lnz(n, m)) {
+              E.addL(n, m, k * dW[m], A);
+              dWb[m] += k * E.L(n, m);
+            }
+          }
+          evb[0] += qab * c * LdW;
+          evb[n] += -qab * c * LdW + qa2b * c * LdW;
+          evb[6] -= qa2b * c * LdW;
+        }
+      }
+      // Step 8
+      const double qx1 = 0.5 * dmax(E.ev[6], 0.0) * dtodx;
+      const double qx2 = -0.5 * dmin(E.ev[0], 0.0) * dtodx;
+      double qx1b = 0.0, qx2b = 0.0;
+#pragma unroll
+      for (int m = 0; m < 7; m++) {
+        qx1b -= out_lb[m] * dW[m];
+        qx2b += out_rb[m] * dW[m];
+        dWb[m] += qx2 * out_rb[m] - qx1 * out_lb[m];
+      }
+      if (E.ev[6] > 0.0) evb[6] += qx1b * 0.5 * dtodx;
+      if (E.ev[0] < 0.0) evb[0] -= qx2b * 0.5 * dtodx;
+    }
+    // Step 7: clamps, then Step 5 (dWm = R da)
+    {
+      double da[7], Wlv0[7], Wrv0[7];
+      chain(da, Wlv0, Wrv0, nullptr, nullptr, nullptr);
+      double dWmb[7];
+#pragma unroll
+      for (int n = 0; n < 7; n++) {
+        const double Wrvb = out_lb[n] + dWb[n];
+        const double Wlvb = out_rb[n] - dWb[n];
+        // Wlv = MIN(MAX(w0,wm), MAX(MIN(w0,wm), Wlv0))
+        double lob = 0, hib = 0, s1b = 0, Wlv0b = 0, Wrv0b = 0;
+        {
+          const double lo = dmin(w0[n], wm[n]), hi = dmax(w0[n], wm[n]);
+          const double s1 = dmax(lo, Wlv0[n]);
+          if (hi < s1) hib = Wlvb; else s1b = Wlvb;
+          if (lo > Wlv0[n]) lob = s1b; else Wlv0b = s1b;
+          if (w0[n] < wm[n]) w0b[n] += lob; else wmb[n] += lob;
+          if (w0[n] > wm[n]) w0b[n] += hib; else wmb[n] += hib;
+        }
+        {
+          lob = hib = s1b = 0;
+          const double lo = dmin(w0[n], wp[n]), hi = dmax(w0[n], wp[n]);
+          const double s1 = dmax(lo, Wrv0[n]);
+          if (hi < s1) hib = Wrvb; else s1b = Wrvb;
+          if (lo > Wrv0[n]) lob = s1b; else Wrv0b = s1b;
+          if (w0[n] < wp[n]) w0b[n] += lob; else wpb[n] += lob;
+          if (w0[n] > wp[n]) w0b[n] += hib; else wpb[n] += hib;
+        }
+        w0b[n] += Wlv0b + Wrv0b;
+        dWmb[n] = 0.5 * (Wrv0b - Wlv0b);
+      }
+#pragma unroll
+      for (int m = 0; m < 7; m++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int n = 0; n < 7; n++)
+          if (prim_Rnz(n, m)) {
+            acc += dWmb[n] * E.R(n, m);
+            E.addR(n, m, dWmb[n] * da[m], A);
+          }
+        dWb[m] = acc;  // reused: the adjoint of the slope da[m]
+      }
+    }
+    // Step 4 (limiter) and Step 3 (projections) per characteristic field n
+    double dWcb[7], dWlb[7], dWrb[7], dWgb[7];
+    for (int m = 0; m < 7; m++) dWcb[m] = dWlb[m] = dWrb[m] = dWgb[m] = 0.0;
+#pragma unroll
+    for (int n = 0; n < 7; n++) {
+      double ac, al, ar, ag;
+      project(n, ac, al, ar, ag);
+      double acb = 0.0, alb = 0.0, arb = 0.0, agb = 0.0;
+      if (al * ar > 0.0) {
+        const double ls1 = dmin(fabs(al), fabs(ar));
+        const double ls2 = dmin(0.5 * fabs(ac), fabs(ag));
+        const double sg = dsign(ac);
+        double ls1b = 0.0, ls2b = 0.0;
+        if (2.0 * ls1 < ls2) ls1b = 2.0 * sg * dWb[n]; else ls2b = sg * dWb[n];
+        if (fabs(al) < fabs(ar)) alb += ls1b * sgnabs(al);
+        else arb += ls1b * sgnabs(ar);
+        if (0.5 * fabs(ac) < fabs(ag)) acb += ls2b * 0.5 * sgnabs(ac);
+        else agb += ls2b * sgnabs(ag);
+      }
+#pragma unroll
+      for (int m = 0; m < 7; m++) {
         if (!prim_Lnz(n, m)) continue;
-        E.addL(n, m, dacb[n] * dWc[m] + dalb[n] * dWl[m] + darb[n] * dWr[m] + dagb[n] * dWg[m], A);
+        double dc, dl, dr, dg;
+        diffs(m, dc, dl, dr, dg);
+        E.addL(n, m, acb * dc + alb * dl + arb * dr + agb * dg, A);
         const double L = E.L(n, m);
-        c_ += dacb[n] * L;
-        l_ += dalb[n] * L;
-        r_ += darb[n] * L;
-        g_ += dagb[n] * L;
+        dWcb[m] += acb * L;
+        dWlb[m] += alb * L;
+        dWrb[m] += arb * L;
+        dWgb[m] += agb * L;
       }
-      double dWlb = l_, dWrb = r_;
-      if (dWl[m] * dWr[m] > 0.0) {
-        const double S = dWl[m] + dWr[m];
-        dWlb += g_ * 2.0 * dWr[m] * dWr[m] / (S * S);
-        dWrb += g_ * 2.0 * dWl[m] * dWl[m] / (S * S);
+    }
+    // Step 2
+#pragma unroll
+    for (int m = 0; m < 7; m++) {
+      const double dl = w0[m] - wm[m], dr = wp[m] - w0[m];
+      double lb = dWlb[m], rb = dWrb[m];
+      if (dl * dr > 0.0) {
+        const double S = dl + dr;
+        lb += dWgb[m] * 2.0 * dr * dr / (S * S);
+        rb += dWgb[m] * 2.0 * dl * dl / (S * S);
       }
-      wpb[m] += c_ + dWrb;
-      wmb[m] -= c_ + dWlb;
-      w0b[m] += dWlb - dWrb;
+      wpb[m] += dWcb[m] + rb;
+      wmb[m] -= dWcb[m] + lb;
+      w0b[m] += lb - rb;
     }
     // eigensystem of W[i] (d, vx, p, Bxc, by, bz)
     double inb[6] = {0, 0, 0, 0, 0, 0};
