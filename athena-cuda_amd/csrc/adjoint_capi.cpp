@@ -1,10 +1,10 @@
 // adjoint_capi.cpp -- host runtime of the differentiation form (include/
 // athmi_adjoint.h): a fixed-dt rollout of N CTU+CT steps of one instance
 // (set_bvals, then N times integrate_2d + set_bvals) with a checkpoint every
-// K steps, and its reverse: per segment of K steps the states are recomputed
-// from the checkpoint, then each step is reversed (its forward scratch is
-// rebuilt from the stored input state by the forward kernels, and the adjoint
-// kernels run the stages backwards).
+// K steps, and its reverse: per segment of K steps the steps are re-run from
+// the checkpoint, keeping each step's input state and forward scratch
+// (predictor face states and fluxes, predictor corner emf, corrector fluxes),
+// then each step is reversed by the adjoint kernels, last step first.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,6 +21,8 @@ namespace {
 // forward scratch of one step: predictor, predictor corner emf, corrector fluxes
 const int kSeqForwardScratch[] = {SK_BEGIN_ONCE, SK_PRED_X1, SK_PRED_X2, SK_CORNER_P, SK_CORR,
                                   SK_CORR2};
+// the rest of integrate_2d after kSeqForwardScratch
+const int kSeqFinishStep[] = {SK_CORNER_C, SK_FINAL, SK_GHOST_CARRY, SK_END_ONCE};
 const int kSeqReverse[] = {AK_FOLD_OUT,  AK_ZERO,       AK_UPDATE,   AK_CORNER_C, AK_RIEMANN_C,
                            AK_HALF,      AK_TRANSVERSE, AK_CORNER_P, AK_EMF_CC,   AK_PRED_X1,
                            AK_PLM_X1,    AK_PRED_X2,    AK_PLM_X2};
@@ -38,7 +40,9 @@ struct athmi_adjoint {
   double* seg = nullptr;
   double* adj[2] = {nullptr, nullptr};
   double* ab = nullptr;
+  double* segsc = nullptr;   // forward scratch of each step of a segment
   size_t state_doubles = 0;  // NSTATE * ps
+  size_t scratch_doubles = 0;  // NSCRATCH * ps
 };
 
 static int adj_issue(athmi_adjoint* h, const int* seq, size_t n) {
@@ -104,7 +108,9 @@ int athmi_adjoint_create(const athmi_desc* desc, double dt, int nsteps, int ever
   h->nck = (nsteps + h->every - 1) / h->every;
   const Inst& I = h->g->hinst[0];
   h->state_doubles = (size_t)NSTATE * I.ps;
-  const size_t total = h->state_doubles * (h->nck + h->every + 2) + (size_t)NADJ * I.ps;
+  h->scratch_doubles = (size_t)NSCRATCH * I.ps;
+  const size_t total = h->state_doubles * (h->nck + h->every + 2) + (size_t)NADJ * I.ps +
+                       h->scratch_doubles * h->every;
   if (hipMalloc(&h->mem, total * sizeof(double)) != hipSuccess) {
     athmi_destroy(h->g);
     delete h;
@@ -115,6 +121,7 @@ int athmi_adjoint_create(const athmi_desc* desc, double dt, int nsteps, int ever
   h->adj[0] = h->seg + h->state_doubles * h->every;
   h->adj[1] = h->adj[0] + h->state_doubles;
   h->ab = h->adj[1] + h->state_doubles;
+  h->segsc = h->ab + (size_t)NADJ * I.ps;
   Clock c;
   if ((rc = h->g->read_clock(0, c)) || (rc = athmi_set_clock(h->g, 0, dt, 0.0, 0))) {
     athmi_adjoint_destroy(h);
@@ -172,20 +179,29 @@ int athmi_adjoint_backward(athmi_adjoint* h, const double* gradN, double* grad0)
   a.x2_seg = h->g->bl.shape.x2_seg;
   for (int s = h->nck - 1; s >= 0; s--) {
     const int n0 = s * h->every, n1 = std::min(n0 + h->every, h->nsteps);
-    // states n0 .. n1-1 of this segment from its checkpoint
+    // re-run the segment from its checkpoint, keeping each step's input state
+    // and its forward scratch (predictor states and fluxes, predictor corner
+    // emf, corrector fluxes: taken before the corrector corners overwrite the
+    // emf plane)
     const double* ck = h->ck + (size_t)s * h->state_doubles;
-    if ((rc = copy_state(h, h->seg, ck)) || (rc = copy_state(h, live_state(h), ck))) return rc;
-    for (int m = n0 + 1; m < n1; m++) {
-      if ((rc = rollout_step(h))) return rc;
-      if ((rc = copy_state(h, h->seg + (size_t)(m - n0) * h->state_doubles, live_state(h))))
-        return rc;
-    }
-    for (int m = n1 - 1; m >= n0; m--) {
-      double* st = live_state(h);
-      if ((rc = copy_state(h, st, h->seg + (size_t)(m - n0) * h->state_doubles))) return rc;
+    if ((rc = copy_state(h, live_state(h), ck))) return rc;
+    for (int m = n0; m < n1; m++) {
+      double* st_m = h->seg + (size_t)(m - n0) * h->state_doubles;
+      if ((rc = copy_state(h, st_m, live_state(h)))) return rc;
       if ((rc = adj_issue(h, kSeqForwardScratch, sizeof(kSeqForwardScratch) / sizeof(int))))
         return rc;
-      a.st = st;
+      HIP_TRY(hipMemcpyAsync(h->segsc + (size_t)(m - n0) * h->scratch_doubles, a.g.scratch,
+                             h->scratch_doubles * sizeof(double), hipMemcpyDeviceToDevice,
+                             h->g->stream));
+      if (m + 1 < n1) {
+        if ((rc = adj_issue(h, kSeqFinishStep, sizeof(kSeqFinishStep) / sizeof(int)))) return rc;
+        h->cur ^= 1;  // k_end swaps the buffers
+        HIP_TRY(h->g->launch(&h->g->bl, SK_GHOST_CUR, h->g->stream));
+      }
+    }
+    for (int m = n1 - 1; m >= n0; m--) {
+      a.st = h->seg + (size_t)(m - n0) * h->state_doubles;
+      a.sc = h->segsc + (size_t)(m - n0) * h->scratch_doubles;
       a.ain = ain;
       a.aout = aout;
       for (int k : kSeqReverse) HIP_TRY(athmi_adjoint_launch(&a, k, h->g->stream));
