@@ -2,9 +2,11 @@
 // athmi_adjoint.h): a fixed-dt rollout of N CTU+CT steps of one instance
 // (set_bvals, then N times integrate_2d + set_bvals) with a checkpoint every
 // K steps, and its reverse: per segment of K steps the steps are re-run from
-// the checkpoint, keeping each step's input state and forward scratch
-// (predictor face states and fluxes, predictor corner emf, corrector fluxes),
-// then each step is reversed by the adjoint kernels, last step first.
+// the checkpoint, each step reading its input state from and writing its
+// output state and forward scratch (predictor face states, conserved and
+// primitive, and fluxes, predictor corner emf, corrector fluxes) into the
+// segment's per-step slots; then each step is reversed by the adjoint
+// kernels, last step first.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,9 +25,16 @@ const int kSeqForwardScratch[] = {SK_BEGIN_ONCE, SK_PRED_X1, SK_PRED_X2, SK_CORN
                                   SK_CORR2};
 // the rest of integrate_2d after kSeqForwardScratch
 const int kSeqFinishStep[] = {SK_CORNER_C, SK_FINAL, SK_GHOST_CARRY, SK_END_ONCE};
+const int kSeqGhostCur[] = {SK_GHOST_CUR};
 const int kSeqReverse[] = {AK_FOLD_OUT,  AK_ZERO,       AK_UPDATE,   AK_CORNER_C, AK_RIEMANN_C,
                            AK_HALF,      AK_TRANSVERSE, AK_CORNER_P, AK_EMF_CC,   AK_PRED_X1,
                            AK_PLM_X1,    AK_PRED_X2,    AK_PLM_X2};
+// a slot of the segment's forward scratch: the step's NSCRATCH planes, its
+// predictor corner emf (the corrector's corner kernel overwrites S_EMF) and
+// the predictors' NPRIMS primitive face states
+constexpr int kSlotEmf = NSCRATCH;
+constexpr int kSlotPrims = NSCRATCH + 1;
+constexpr int kSlotPlanes = NSCRATCH + 1 + NPRIMS;
 
 }  // namespace
 
@@ -37,13 +46,25 @@ struct athmi_adjoint {
   bool have_forward = false;
   double* mem = nullptr;    // checkpoints | segment states | adjoint state x2 | adjoint planes
   double* ck = nullptr;
-  double* seg = nullptr;
+  double* seg = nullptr;     // K + 1 states: the input of each step of a segment
   double* adj[2] = {nullptr, nullptr};
   double* ab = nullptr;
-  double* segsc = nullptr;   // forward scratch of each step of a segment
+  double* segsc = nullptr;   // K slots of kSlotPlanes: forward scratch of each step
   size_t state_doubles = 0;  // NSTATE * ps
-  size_t scratch_doubles = 0;  // NSCRATCH * ps
+  size_t slot_doubles = 0;   // kSlotPlanes * ps
+  // 2K device descriptors of the re-run: step m of a segment whose first step
+  // reads state buffer p is dslot[p * K + m] (state buffers seg[m], seg[m+1],
+  // scratch slot m)
+  Inst* dslot = nullptr;
 };
+
+// launch a kernel sequence of the instance through the descriptor ins
+static int issue_on(athmi_adjoint* h, const Inst* ins, const int* seq, size_t n) {
+  BatchLaunch bl = h->g->bl;
+  bl.insts = ins;
+  for (size_t k = 0; k < n; k++) HIP_TRY(h->g->launch(&bl, seq[k], h->g->stream));
+  return ATHMI_OK;
+}
 
 static int adj_issue(athmi_adjoint* h, const int* seq, size_t n) {
   return issue_seq(h->g, seq, n);
@@ -108,20 +129,35 @@ int athmi_adjoint_create(const athmi_desc* desc, double dt, int nsteps, int ever
   h->nck = (nsteps + h->every - 1) / h->every;
   const Inst& I = h->g->hinst[0];
   h->state_doubles = (size_t)NSTATE * I.ps;
-  h->scratch_doubles = (size_t)NSCRATCH * I.ps;
-  const size_t total = h->state_doubles * (h->nck + h->every + 2) + (size_t)NADJ * I.ps +
-                       h->scratch_doubles * h->every;
-  if (hipMalloc(&h->mem, total * sizeof(double)) != hipSuccess) {
-    athmi_destroy(h->g);
-    delete h;
+  h->slot_doubles = (size_t)kSlotPlanes * I.ps;
+  const size_t total = h->state_doubles * (h->nck + h->every + 3) + (size_t)NADJ * I.ps +
+                       h->slot_doubles * h->every;
+  if (hipMalloc(&h->mem, total * sizeof(double)) != hipSuccess ||
+      hipMalloc(&h->dslot, sizeof(Inst) * 2 * h->every) != hipSuccess) {
+    athmi_adjoint_destroy(h);
     return fail(ATHMI_ENOMEM, "athmi_adjoint_create: no room for checkpoints");
   }
   h->ck = h->mem;
   h->seg = h->ck + h->state_doubles * h->nck;
-  h->adj[0] = h->seg + h->state_doubles * h->every;
+  h->adj[0] = h->seg + h->state_doubles * (h->every + 1);
   h->adj[1] = h->adj[0] + h->state_doubles;
   h->ab = h->adj[1] + h->state_doubles;
   h->segsc = h->ab + (size_t)NADJ * I.ps;
+  std::vector<Inst> slots(2 * h->every, I);
+  for (int p = 0; p < 2; p++)
+    for (int m = 0; m < h->every; m++) {
+      Inst& S = slots[p * h->every + m];
+      const int cur = p ^ (m & 1);
+      S.state[cur] = h->seg + (size_t)m * h->state_doubles;
+      S.state[cur ^ 1] = h->seg + (size_t)(m + 1) * h->state_doubles;
+      S.scratch = h->segsc + (size_t)m * h->slot_doubles;
+      S.prims = S.scratch + (size_t)kSlotPrims * I.ps;
+    }
+  if (hipMemcpy(h->dslot, slots.data(), sizeof(Inst) * slots.size(), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    athmi_adjoint_destroy(h);
+    return fail(ATHMI_EHIP, "athmi_adjoint_create: descriptor upload failed");
+  }
   Clock c;
   if ((rc = h->g->read_clock(0, c)) || (rc = athmi_set_clock(h->g, 0, dt, 0.0, 0))) {
     athmi_adjoint_destroy(h);
@@ -136,6 +172,7 @@ int athmi_adjoint_destroy(athmi_adjoint* h) {
   if (!h) return ATHMI_OK;
   if (h->g && h->g->stream) (void)hipStreamSynchronize(h->g->stream);
   if (h->mem) (void)hipFree(h->mem);
+  if (h->dslot) (void)hipFree(h->dslot);
   if (h->g) athmi_destroy(h->g);
   delete h;
   return ATHMI_OK;
@@ -173,35 +210,36 @@ int athmi_adjoint_backward(athmi_adjoint* h, const double* gradN, double* grad0)
   if ((rc = dense_to_planes(h, aout, gradN))) return rc;
   AdjArgs a;
   a.g = h->g->hinst[0];
-  a.sc = a.g.scratch;
   a.ab = h->ab;
   a.dt = h->dt;
-  a.x2_seg = h->g->bl.shape.x2_seg;
+  a.ain = ain;
+  HIP_TRY(athmi_adjoint_launch(&a, AK_CLEAR, h->g->stream));
+  const size_t ps = a.g.ps;
   for (int s = h->nck - 1; s >= 0; s--) {
     const int n0 = s * h->every, n1 = std::min(n0 + h->every, h->nsteps);
-    // re-run the segment from its checkpoint, keeping each step's input state
-    // and its forward scratch (predictor states and fluxes, predictor corner
-    // emf, corrector fluxes: taken before the corrector corners overwrite the
-    // emf plane)
-    const double* ck = h->ck + (size_t)s * h->state_doubles;
-    if ((rc = copy_state(h, live_state(h), ck))) return rc;
+    // re-run the segment from its checkpoint into the per-step slots
+    const Inst* slot = h->dslot + (size_t)h->cur * h->every;
+    if ((rc = copy_state(h, h->seg, h->ck + (size_t)s * h->state_doubles))) return rc;
     for (int m = n0; m < n1; m++) {
-      double* st_m = h->seg + (size_t)(m - n0) * h->state_doubles;
-      if ((rc = copy_state(h, st_m, live_state(h)))) return rc;
-      if ((rc = adj_issue(h, kSeqForwardScratch, sizeof(kSeqForwardScratch) / sizeof(int))))
+      const Inst* ins = slot + (m - n0);
+      double* sc = h->segsc + (size_t)(m - n0) * h->slot_doubles;
+      if ((rc = issue_on(h, ins, kSeqForwardScratch, sizeof(kSeqForwardScratch) / sizeof(int))))
         return rc;
-      HIP_TRY(hipMemcpyAsync(h->segsc + (size_t)(m - n0) * h->scratch_doubles, a.g.scratch,
-                             h->scratch_doubles * sizeof(double), hipMemcpyDeviceToDevice,
-                             h->g->stream));
+      HIP_TRY(hipMemcpyAsync(sc + kSlotEmf * ps, sc + S_EMF * ps, ps * sizeof(double),
+                             hipMemcpyDeviceToDevice, h->g->stream));
       if (m + 1 < n1) {
-        if ((rc = adj_issue(h, kSeqFinishStep, sizeof(kSeqFinishStep) / sizeof(int)))) return rc;
-        h->cur ^= 1;  // k_end swaps the buffers
-        HIP_TRY(h->g->launch(&h->g->bl, SK_GHOST_CUR, h->g->stream));
+        if ((rc = issue_on(h, ins, kSeqFinishStep, sizeof(kSeqFinishStep) / sizeof(int))))
+          return rc;
+        h->cur ^= 1;  // k_end swaps the buffers: the output is seg[m + 1]
+        if ((rc = issue_on(h, ins, kSeqGhostCur, 1))) return rc;
       }
     }
     for (int m = n1 - 1; m >= n0; m--) {
+      const double* sc = h->segsc + (size_t)(m - n0) * h->slot_doubles;
       a.st = h->seg + (size_t)(m - n0) * h->state_doubles;
-      a.sc = h->segsc + (size_t)(m - n0) * h->scratch_doubles;
+      a.sc = sc;
+      a.emfp = sc + kSlotEmf * ps;
+      a.pw = sc + kSlotPrims * ps;
       a.ain = ain;
       a.aout = aout;
       for (int k : kSeqReverse) HIP_TRY(athmi_adjoint_launch(&a, k, h->g->stream));

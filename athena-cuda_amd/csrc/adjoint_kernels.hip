@@ -2,9 +2,10 @@
 // followed by set_bvals (set_bvals.c:114-1031), for one instance.
 //
 // Given the adjoint of the step's output state (aout) and the forward scratch
-// of the step recomputed from its input state S^n (predictor face states and
-// fluxes, predictor corner emf, corrector fluxes), the kernels below run the
-// step's stages in reverse and accumulate the adjoint of S^n into ain:
+// of the step recomputed from its input state S^n (predictor face states,
+// conserved and primitive, and fluxes, predictor corner emf, corrector
+// fluxes), the kernels below run the step's stages in reverse and accumulate
+// the adjoint of S^n into ain:
 //   fold        set_bvals: each ghost's adjoint is added to the cell it copies
 //   update      Steps 9, 11a, 11b, 13 -> corrector fluxes, corner emf, identity
 //   corner<C>   Step 9 corner emf -> corrector fluxes' By/Bz, emf3 at t^{n+1/2}
@@ -13,7 +14,7 @@
 //   transverse  Steps 5, 6 and the half-step face fields of Step 4
 //   corner<P>   Step 4 corner emf -> predictor fluxes' By/Bz, emf3 at t^n
 //   emf_cc      Step 3
-//   pred_x1/x2  Steps 1c-1e / 2c-2e: sources, prim->cons, predictor fluxes
+//   pred<X1/X2> Steps 1c-1e / 2c-2e: sources, prim->cons, predictor fluxes
 //   plm_x1/x2   Steps 1a-1b / 2a-2b: cons->prim and the PLM reconstruction
 // Nonlinear stages recompute their forward values with the reference's
 // evaluation order (this file is built with -ffp-contract=off, like the exact
@@ -78,8 +79,11 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_fold(AdjArgs a, int which) 
   }
 }
 
-__global__ void __launch_bounds__(256) ak_zero(AdjArgs a) {
-  const size_t n_in = (size_t)NSTATE * a.g.ps, n_ab = (size_t)NADJ * a.g.ps;
+// ALL: every adjoint plane (start of a reverse pass); otherwise ain and the
+// NACC accumulated planes (every step)
+__global__ void __launch_bounds__(256) ak_zero(AdjArgs a, int all) {
+  const size_t n_in = (size_t)NSTATE * a.g.ps;
+  const size_t n_ab = (size_t)(all ? NADJ : NACC) * a.g.ps;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_in + n_ab;
        e += (size_t)gridDim.x * blockDim.x) {
     if (e < n_in) a.ain[e] = 0.0;
@@ -88,11 +92,13 @@ __global__ void __launch_bounds__(256) ak_zero(AdjArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Steps 9, 11a, 11b, 13 (:473-534) over [js,je+1] x [is,ie+1]
+// Steps 9, 11a, 11b, 13 (:473-534) over [js,je+1] x [is,ie+1]; the corrector
+// flux adjoints are stored over [js-1,je+1] x [is-1,ie+1], the faces the
+// corrector Riemann reverse reads (zero where no update reads the flux)
 __global__ void __launch_bounds__(kTileX* kTileY) ak_update(AdjArgs a) {
   const Inst& g = a.g;
   int j, i;
-  if (!range_ij(g.js, g.je + 1, g.is, g.ie + 1, j, i)) return;
+  if (!range_ij(g.js - 1, g.je + 1, g.is - 1, g.ie + 1, j, i)) return;
   const size_t ps = g.ps;
   const int pitch = g.pitch;
   const double dtodx1 = a.dt / g.dx1, dtodx2 = a.dt / g.dx2;
@@ -107,24 +113,25 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_update(AdjArgs a) {
     if (q == F_B3C) v += ao[F_B3I * ps + kk];
     return v;
   };
-  if (j <= g.je && i <= g.ie) {
+  if (j >= g.js && j <= g.je && i >= g.is && i <= g.ie) {
     const int q_id[6] = {F_D, F_M1, F_M2, F_M3, F_E, F_B3C};
     for (int n = 0; n < 6; n++) a.ain[q_id[n] * ps + k] += cb(q_id[n], j, i);
   }
   // corrector fluxes; the B2c update of 11a and the B1c update of 11b are
   // overwritten by Step 13, so those components get no adjoint there
-  if (j <= g.je) {
+  if (i >= g.is) {
     const int q1[7] = {F_D, F_M1, F_M2, F_M3, F_E, -1, F_B3C};
     double* F = a.ab + A_F1C * ps;
     for (int n = 0; n < 7; n++)
       F[n * ps + k] = (q1[n] < 0) ? 0.0 : dtodx1 * (cb(q1[n], j, i) - cb(q1[n], j, i - 1));
   }
-  if (i <= g.ie) {
+  if (j >= g.js) {
     const int q2[7] = {F_D, F_M2, F_M3, F_M1, F_E, F_B3C, -1};
     double* F = a.ab + A_F2C * ps;
     for (int n = 0; n < 7; n++)
       F[n * ps + k] = (q2[n] < 0) ? 0.0 : dtodx2 * (cb(q2[n], j, i) - cb(q2[n], j - 1, i));
   }
+  if (j < g.js || i < g.is) return;
   // adjoint of the updated face fields including B1c, B2c = face averages
   auto b1t = [&](int jj, int ii) -> double {
     if (jj < g.js || jj > g.je || ii < g.is || ii > g.ie + 1) return 0.0;
@@ -195,15 +202,14 @@ struct CorrCtx {
   const Inst& g;
   const double* st;
   const double* sc;
+  const double* emf;  // predictor corner emf
   size_t ps;
   int pitch;
   double hdtodx1, hdtodx2;
   __device__ double b1h(uint32_t kk) const {
-    const double* emf = sc + S_EMF * ps;
     return st[F_B1I * ps + kk] - hdtodx2 * (emf[kk + pitch] - emf[kk]);
   }
   __device__ double b2h(uint32_t kk) const {
-    const double* emf = sc + S_EMF * ps;
     return st[F_B2I * ps + kk] + hdtodx1 * (emf[kk + 1] - emf[kk]);
   }
   __device__ double s(int f, uint32_t kk) const { return st[f * ps + kk]; }
@@ -283,7 +289,7 @@ struct CorrCtx {
 
 __device__ __forceinline__ CorrCtx corr_ctx(const AdjArgs& a) {
   const double dtodx1 = a.dt / a.g.dx1, dtodx2 = a.dt / a.g.dx2;
-  return CorrCtx{a.g, a.st, a.sc, a.g.ps, a.g.pitch, 0.5 * dtodx1, 0.5 * dtodx2};
+  return CorrCtx{a.g, a.st, a.sc, a.emfp, a.g.ps, a.g.pitch, 0.5 * dtodx1, 0.5 * dtodx2};
 }
 
 // Steps 8b, 8c (:438-464) over [js-1,je+1] x [is-1,ie+1]
@@ -482,41 +488,49 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_emf_cc(AdjArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Predictor faces: the face state pair (PLM outputs wl, wr + the 1/2 dt MHD
-// source on the transverse field + prim->cons) and its Riemann flux, reversed.
-// bplane: the normal face field; mplane: the momentum of the source's
-// velocity; src_slot: component of Prim7 receiving the source (by for x1, bz
-// for x2). Returns the adjoints of the PLM outputs in wlb, wrb.
-__device__ __forceinline__ void pred_face_b(const AdjArgs& a, uint32_t k, uint32_t stride,
-                                            Prim7 wl, Prim7 wr, double dx, int bplane,
-                                            int mplane, bool x1, const Cons7& Fb, Cons7 Ulb,
-                                            Cons7 Urb, Prim7& wlb, Prim7& wrb) {
+// Steps 1c-1e / 2c-2e (:156-180, :216-241) reversed per predictor face: the
+// Riemann flux, prim->cons and the 1/2 dt MHD source on the transverse field
+// (By for x1 faces, Bz for x2 faces), from the face states the forward pass
+// stored. X1: faces [js-2,je+2] x [is-1,ie+2]; x2: [js-1,je+2] x [is-2,ie+2].
+// Writes the adjoints of the PLM outputs Wl, Wr (the source only adds to them).
+template <bool X1>
+__global__ void __launch_bounds__(kTileX* kTileY) ak_pred(AdjArgs a) {
   const Inst& g = a.g;
+  int j, i;
+  if (X1) {
+    if (!range_ij(g.js - 2, g.je + 2, g.is - 1, g.ie + 2, j, i)) return;
+  } else {
+    if (!range_ij(g.js - 1, g.je + 2, g.is - 2, g.ie + 2, j, i)) return;
+  }
   const size_t ps = g.ps;
-  const double* st = a.st;
+  const uint32_t stride = X1 ? 1u : (uint32_t)g.pitch;
+  const uint32_t k = (uint32_t)(j * g.pitch + i);
   const EosC eos{g.gamma, g.g1, g.g2};
+  const double* st = a.st;
+  const int bplane = X1 ? F_B1I : F_B2I;  // normal face field
+  const int mplane = X1 ? F_M2 : F_M1;    // momentum of the source's velocity
+  const double b0 = st[bplane * ps + k];
+  Cons7 Ulb = ld7(a.ab + (X1 ? A_UL1 : A_UL2) * ps, ps, k);
+  Cons7 Urb = ld7(a.ab + (X1 ? A_UR1 : A_UR2) * ps, ps, k);
+  Prim7 wlb = zero_prim(), wrb = zero_prim();
+  double b0b = 0.0;
+  {
+    const Prim7 wl = ld7p(a.pw + (X1 ? P_WL1 : P_WL2) * ps, ps, k);
+    const Prim7 wr = ld7p(a.pw + (X1 ? P_WR1 : P_WR2) * ps, ps, k);
+    riemann_b(g.flux, ld7(a.sc + (X1 ? S_UL1 : S_UL2) * ps, ps, k),
+              ld7(a.sc + (X1 ? S_UR1 : S_UR2) * ps, ps, k), wl, wr, b0, eos,
+              ld7(a.ab + (X1 ? A_F1 : A_F2) * ps, ps, k), Ulb, Urb, wlb, wrb, b0b);
+    prim_to_cons_b(wl, b0, eos, Ulb, wlb, b0b);
+    prim_to_cons_b(wr, b0, eos, Urb, wrb, b0b);
+  }
+  // source: W.bt += 1/2 dt (M/d)[cell] (B[face+1] - B[face]) / dx, left state
+  // from the cell behind the face, right state from the cell ahead
+  const double dx = X1 ? g.dx1 : g.dx2;
   const double hdt = 0.5 * a.dt;
-  const double bm = st[bplane * ps + k - stride], b0 = st[bplane * ps + k];
-  const double bp = st[bplane * ps + k + stride];
+  const double bm = st[bplane * ps + k - stride], bp = st[bplane * ps + k + stride];
   const double dl = st[F_D * ps + k - stride], dr = st[F_D * ps + k];
   const double vl = st[mplane * ps + k - stride] / dl, vr = st[mplane * ps + k] / dr;
-  double srcl = vl * (b0 - bm) / dx;
-  double srcr = vr * (bp - b0) / dx;
-  if (x1) {
-    wl.by = wl.by + hdt * srcl;
-    wr.by = wr.by + hdt * srcr;
-  } else {
-    wl.bz = wl.bz + hdt * srcl;
-    wr.bz = wr.bz + hdt * srcr;
-  }
-  const Cons7 ul = prim_to_cons(wl, b0, eos), ur = prim_to_cons(wr, b0, eos);
-  wlb = zero_prim();
-  wrb = zero_prim();
-  double b0b = 0.0;
-  riemann_b(g.flux, ul, ur, wl, wr, b0, eos, Fb, Ulb, Urb, wlb, wrb, b0b);
-  prim_to_cons_b(wl, b0, eos, Ulb, wlb, b0b);
-  prim_to_cons_b(wr, b0, eos, Urb, wrb, b0b);
-  const double slb = hdt * (x1 ? wlb.by : wlb.bz), srb = hdt * (x1 ? wrb.by : wrb.bz);
+  const double slb = hdt * (X1 ? wlb.by : wlb.bz), srb = hdt * (X1 ? wrb.by : wrb.bz);
   const double dblb = slb * vl / dx, vlb = slb * (b0 - bm) / dx;
   const double dbrb = srb * vr / dx, vrb = srb * (bp - b0) / dx;
   ADD(a.ain + bplane * ps, k - stride, -dblb);
@@ -526,105 +540,8 @@ __device__ __forceinline__ void pred_face_b(const AdjArgs& a, uint32_t k, uint32
   ADD(a.ain + F_D * ps, k - stride, -vlb * vl / dl);
   ADD(a.ain + mplane * ps, k, vrb / dr);
   ADD(a.ain + F_D * ps, k, -vrb * vr / dr);
-}
-
-__device__ __forceinline__ Prim7 lds_get(double (*s)[kX1Block], int t) {
-  return Prim7{s[0][t], s[1][t], s[2][t], s[3][t], s[4][t], s[5][t], s[6][t]};
-}
-__device__ __forceinline__ void lds_set(double (*s)[kX1Block], int t, const Prim7& w) {
-  s[0][t] = w.d; s[1][t] = w.vx; s[2][t] = w.vy; s[3][t] = w.vz;
-  s[4][t] = w.p; s[5][t] = w.by; s[6][t] = w.bz;
-}
-
-// Steps 1c-1e reversed: one row segment of x1 faces per workgroup (the
-// forward k_pred_x1 layout: 256 cells -> 253 faces)
-__global__ void __launch_bounds__(kX1Block) ak_pred_x1(AdjArgs a) {
-  __shared__ double sw[7][kX1Block];
-  __shared__ double swl[7][kX1Block];
-  __shared__ double swr[7][kX1Block];
-  const Inst& g = a.g;
-  const int tid = threadIdx.x;
-  const int nseg = (g.nx1 + 4 + kX1Faces - 1) / kX1Faces;
-  const int row = (int)blockIdx.x / nseg, seg = (int)blockIdx.x - row * nseg;
-  const int j = g.js - 2 + row;
-  const int f0 = g.is - 1 + seg * kX1Faces;
-  const int cell = f0 - 2 + tid;
-  const double dtodx1 = a.dt / g.dx1;
-  const EosC eos{g.gamma, g.g1, g.g2};
-  const double* st = a.st;
-  const size_t ps = g.ps;
-  const uint32_t k = (uint32_t)(j * g.pitch + cell);
-  double bxc = 0.0;
-  if (cell >= g.is - 3 && cell <= g.ie + 3) {
-    const Cons7 u{st[F_D * ps + k], st[F_M1 * ps + k], st[F_M2 * ps + k], st[F_M3 * ps + k],
-                  st[F_E * ps + k], st[F_B2C * ps + k], st[F_B3C * ps + k]};
-    bxc = st[F_B1C * ps + k];
-    lds_set(sw, tid, cons_to_prim(u, bxc, eos));
-  }
-  __syncthreads();
-  if (tid >= 1 && tid <= kX1Block - 2 && cell >= g.is - 2 && cell <= g.ie + 2) {
-    Prim7 wl, wr;
-    plm_cell(lds_get(sw, tid - 1), lds_get(sw, tid), lds_get(sw, tid + 1), bxc, dtodx1, eos, wl,
-             wr);
-    lds_set(swl, tid + 1, wl);
-    lds_set(swr, tid, wr);
-  }
-  __syncthreads();
-  if (tid >= 2 && tid <= kX1Block - 2 && cell <= g.ie + 2) {
-    Prim7 wlb, wrb;
-    pred_face_b(a, k, 1u, lds_get(swl, tid), lds_get(swr, tid), g.dx1, F_B1I, F_M2, true,
-                ld7(a.ab + A_F1 * ps, ps, k), ld7(a.ab + A_UL1 * ps, ps, k),
-                ld7(a.ab + A_UR1 * ps, ps, k), wlb, wrb);
-    st7(a.ab + A_WL1 * ps, ps, k, wlb);
-    st7(a.ab + A_WR1 * ps, ps, k, wrb);
-  }
-}
-
-// Steps 2c-2e reversed: lanes own adjacent columns and march down a segment
-// of x2 faces (the forward k_pred_x2 layout)
-__global__ void __launch_bounds__(kX2Block) ak_pred_x2(AdjArgs a) {
-  const Inst& g = a.g;
-  const int lane = threadIdx.x;
-  const int ncb = (g.nx1 + 5 + kX2Block - 1) / kX2Block;
-  const int sg = (int)blockIdx.x / ncb, cb = (int)blockIdx.x - sg * ncb;
-  const int i = g.is - 2 + cb * kX2Block + lane;
-  if (i > g.ie + 2) return;
-  const int jf0 = g.js - 1 + sg * a.x2_seg;
-  const int jf1 = min(jf0 + a.x2_seg - 1, g.je + 2);
-  const double dtodx2 = a.dt / g.dx2;
-  const EosC eos{g.gamma, g.g1, g.g2};
-  const double* st = a.st;
-  const size_t ps = g.ps;
-  const int pitch = g.pitch;
-  auto prim_at = [&](int jj, double& bx) -> Prim7 {
-    const uint32_t k = (uint32_t)(jj * pitch + i);
-    const Cons7 u{st[F_D * ps + k], st[F_M2 * ps + k], st[F_M3 * ps + k], st[F_M1 * ps + k],
-                  st[F_E * ps + k], st[F_B3C * ps + k], st[F_B1C * ps + k]};
-    bx = st[F_B2C * ps + k];
-    return cons_to_prim(u, bx, eos);
-  };
-  double bxa, bxb, bxc;
-  Prim7 wa = prim_at(jf0 - 2, bxa);
-  Prim7 wb = prim_at(jf0 - 1, bxb);
-  Prim7 wc = prim_at(jf0, bxc);
-  Prim7 wl_cur, wr_unused;
-  plm_cell(wa, wb, wc, bxb, dtodx2, eos, wl_cur, wr_unused);
-  for (int j = jf0; j <= jf1; j++) {
-    wa = wb;
-    wb = wc;
-    bxb = bxc;
-    wc = prim_at(j + 1, bxc);
-    Prim7 wl_next, wr_cur;
-    plm_cell(wa, wb, wc, bxb, dtodx2, eos, wl_next, wr_cur);
-    const uint32_t k = (uint32_t)(j * pitch + i);
-    Prim7 wlb, wrb;
-    pred_face_b(a, k, (uint32_t)pitch, wl_cur, wr_cur, g.dx2, F_B2I, F_M1, false,
-                ld7(a.ab + A_F2 * ps, ps, k), ld7(a.ab + A_UL2 * ps, ps, k),
-                ld7(a.ab + A_UR2 * ps, ps, k), wlb, wrb);
-    st7(a.ab + A_WL2 * ps, ps, k, wlb);
-    st7(a.ab + A_WR2 * ps, ps, k, wrb);
-    wl_cur = wl_next;
-  }
+  st7(a.ab + (X1 ? A_WL1 : A_WL2) * ps, ps, k, wlb);
+  st7(a.ab + (X1 ? A_WR1 : A_WR2) * ps, ps, k, wrb);
 }
 
 // Steps 1a-1b / 2a-2b reversed for one cell of [js-2,je+2] x [is-2,ie+2]:
@@ -699,10 +616,11 @@ extern "C" hipError_t athmi_adjoint_launch(const AdjArgs* ap, int kernel, hipStr
                          kernel == AK_FOLD_IN ? 1 : 0);
       break;
     case AK_ZERO:
-      hipLaunchKernelGGL(ak_zero, dim3(2048), dim3(256), 0, s, a);
+    case AK_CLEAR:
+      hipLaunchKernelGGL(ak_zero, dim3(2048), dim3(256), 0, s, a, kernel == AK_CLEAR ? 1 : 0);
       break;
     case AK_UPDATE:
-      hipLaunchKernelGGL(ak_update, grid2(g.js, g.je + 1, g.is, g.ie + 1), b2, 0, s, a);
+      hipLaunchKernelGGL(ak_update, grid2(g.js - 1, g.je + 1, g.is - 1, g.ie + 1), b2, 0, s, a);
       break;
     case AK_CORNER_C:
       hipLaunchKernelGGL(ak_corner<false>, grid2(g.js, g.je + 1, g.is, g.ie + 1), b2, 0, s, a);
@@ -723,16 +641,13 @@ extern "C" hipError_t athmi_adjoint_launch(const AdjArgs* ap, int kernel, hipStr
       hipLaunchKernelGGL(ak_emf_cc, grid2(g.js - 2, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);
       break;
     case AK_PRED_X1:
-      hipLaunchKernelGGL(ak_pred_x1, dim3((unsigned)((g.nx2 + 4) * cdiv(g.nx1 + 4, kX1Faces))),
-                         dim3(kX1Block), 0, s, a);
+      hipLaunchKernelGGL(ak_pred<true>, grid2(g.js - 2, g.je + 2, g.is - 1, g.ie + 2), b2, 0, s, a);
       break;
     case AK_PLM_X1:
       hipLaunchKernelGGL(ak_plm<true>, grid2(g.js - 2, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);
       break;
     case AK_PRED_X2:
-      hipLaunchKernelGGL(ak_pred_x2,
-                         dim3((unsigned)(cdiv(g.nx1 + 5, kX2Block) * cdiv(g.nx2 + 4, a.x2_seg))),
-                         dim3(kX2Block), 0, s, a);
+      hipLaunchKernelGGL(ak_pred<false>, grid2(g.js - 1, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);
       break;
     case AK_PLM_X2:
       hipLaunchKernelGGL(ak_plm<false>, grid2(g.js - 2, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);

@@ -88,6 +88,11 @@ __device__ __forceinline__ void put7(double* base, size_t ps, uint32_t k, const 
   ST(base + 3 * ps, k, u.mz); ST(base + 4 * ps, k, u.e); ST(base + 5 * ps, k, u.by);
   ST(base + 6 * ps, k, u.bz);
 }
+__device__ __forceinline__ void put7(double* base, size_t ps, uint32_t k, const Prim7& w) {
+  ST(base + 0 * ps, k, w.d); ST(base + 1 * ps, k, w.vx); ST(base + 2 * ps, k, w.vy);
+  ST(base + 3 * ps, k, w.vz); ST(base + 4 * ps, k, w.p); ST(base + 5 * ps, k, w.by);
+  ST(base + 6 * ps, k, w.bz);
+}
 __device__ __forceinline__ Cons7 get7(const double* base, size_t ps, uint32_t k) {
   return Cons7{LD(base + 0 * ps, k), LD(base + 1 * ps, k), LD(base + 2 * ps, k), LD(base + 3 * ps, k),
                LD(base + 4 * ps, k), LD(base + 5 * ps, k), LD(base + 6 * ps, k)};
@@ -183,6 +188,10 @@ __global__ void __launch_bounds__(kX1Block) k_pred_x1(const Inst* __restrict__ i
       Prim7 wl, wr;
       double b0;
       build(ul, ur, wl, wr, b0);
+      if (g.prims != nullptr) {
+        put7(g.prims + P_WL1 * ps, ps, k, wl);
+        put7(g.prims + P_WR1 * ps, ps, k, wr);
+      }
       put7(sc + S_UL1 * ps, ps, k, ul);
       put7(sc + S_UR1 * ps, ps, k, ur);
       put7(sc + S_F1 * ps, ps, k, riemann_reload(g.flux, ul, ur, wl, wr, b0, eos, build));
@@ -263,6 +272,10 @@ __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ i
       Prim7 wl, wr;
       double b0;
       build(ul, ur, wl, wr, b0);
+      if (g.prims != nullptr) {
+        put7(g.prims + P_WL2 * ps, ps, k, wl);
+        put7(g.prims + P_WR2 * ps, ps, k, wr);
+      }
       put7(sc + S_UL2 * ps, ps, k, ul);
       put7(sc + S_UR2 * ps, ps, k, ur);
       put7(sc + S_F2 * ps, ps, k, riemann_reload(g.flux, ul, ur, wl, wr, b0, eos, build));

@@ -26,6 +26,10 @@ enum Scratch {
   NSCRATCH = 59
 };
 
+// optional planes of the predictor's primitive face states, after the 1/2 dt
+// MHD source (the Wl, Wr that Steps 1e / 2e hand to the Riemann solver)
+enum PrimSave { P_WL1 = 0, P_WR1 = 7, P_WL2 = 14, P_WR2 = 21, NPRIMS = 28 };
+
 enum BcKind { BC_REFLECT = 1, BC_OUTFLOW = 2, BC_PERIODIC = 4 };
 
 constexpr int kDtSlots = 256;  // partial max slots of the fused dt reduction
@@ -46,6 +50,7 @@ struct Inst {
   double dx1, dx2, gamma, g1, g2;
   double* state[2];   // two ping-pong buffers of NSTATE planes
   double* scratch;    // NSCRATCH planes
+  double* prims;      // NPRIMS planes written by the predictors when non-null
   size_t ps;          // plane stride in doubles
   Clock* clk;
   // first global tile of this instance for each kernel kind
