@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_riemann_c(AdjArgs a) {
     cons_to_prim_b(R, bx, eos, Wrb, Rb, bxb);
     st7(a.ab + A_UL1 * ps, ps, k, Lb);
     st7(a.ab + A_UR1 * ps, ps, k, Rb);
-    ADD(a.ab + A_B1H * ps, k, bxb);
+    a.ab[A_B1H * ps + k] = bxb;
   }
   if (j >= g.js) {
     Cons7 L, R;
@@ -329,11 +329,13 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_riemann_c(AdjArgs a) {
     cons_to_prim_b(R, bx, eos, Wrb, Rb, bxb);
     st7(a.ab + A_UL2 * ps, ps, k, Lb);
     st7(a.ab + A_UR2 * ps, ps, k, Rb);
-    ADD(a.ab + A_B2H * ps, k, bxb);
+    a.ab[A_B2H * ps + k] = bxb;
   }
 }
 
-// Step 7 (:402-431) over [js-1,je+1] x [is-1,ie+1]
+// Step 7 (:402-431) reversed per cell of [js-1,je+1] x [is-1,ie+1]: stores
+// the adjoints of the cell's half-step d, M1, M2 and B1c, B2c; the transverse
+// reverse hands them on to the predictor fluxes and the half-step face fields.
 __global__ void __launch_bounds__(kTileX* kTileY) ak_half(AdjArgs a) {
   const Inst& g = a.g;
   int j, i;
@@ -342,128 +344,137 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_half(AdjArgs a) {
   const int pitch = g.pitch;
   const uint32_t k = (uint32_t)(j * pitch + i);
   const double Eb = a.ab[A_ECH * ps + k];
-  if (Eb == 0.0) return;
-  const CorrCtx c = corr_ctx(a);
-  const double h1 = c.hdtodx1, h2 = c.hdtodx2;
-  const double* F1 = a.sc + S_F1 * ps;
-  const double* F2 = a.sc + S_F2 * ps;
-  const double dhalf = c.s(F_D, k) - h1 * (F1[k + 1] - F1[k]) - h2 * (F2[k + pitch] - F2[k]);
-  const double m1 = c.s(F_M1, k) - h1 * (F1[ps + k + 1] - F1[ps + k]) -
-                    h2 * (F2[3 * ps + k + pitch] - F2[3 * ps + k]);
-  const double m2 = c.s(F_M2, k) - h1 * (F1[2 * ps + k + 1] - F1[2 * ps + k]) -
-                    h2 * (F2[ps + k + pitch] - F2[ps + k]);
-  const double b1c = 0.5 * (c.b1h(k) + c.b1h(k + 1));
-  const double b2c = 0.5 * (c.b2h(k) + c.b2h(k + pitch));
-  const double emf = (b1c * m2 - b2c * m1) / dhalf;
-  const double b1cb = Eb * m2 / dhalf, m2b = Eb * b1c / dhalf;
-  const double b2cb = -Eb * m1 / dhalf, m1b = -Eb * b2c / dhalf;
-  const double dhb = -Eb * emf / dhalf;
-  double* B1hb = a.ab + A_B1H * ps;
-  double* B2hb = a.ab + A_B2H * ps;
-  ADD(B1hb, k, 0.5 * b1cb);
-  ADD(B1hb, k + 1, 0.5 * b1cb);
-  ADD(B2hb, k, 0.5 * b2cb);
-  ADD(B2hb, k + pitch, 0.5 * b2cb);
-  double* F1b = a.ab + A_F1 * ps;
-  double* F2b = a.ab + A_F2 * ps;
-  ADD(a.ain + F_D * ps, k, dhb);
-  ADD(F1b, k + 1, -h1 * dhb);
-  ADD(F1b, k, h1 * dhb);
-  ADD(F2b, k + pitch, -h2 * dhb);
-  ADD(F2b, k, h2 * dhb);
-  ADD(a.ain + F_M1 * ps, k, m1b);
-  ADD(F1b + ps, k + 1, -h1 * m1b);
-  ADD(F1b + ps, k, h1 * m1b);
-  ADD(F2b + 3 * ps, k + pitch, -h2 * m1b);
-  ADD(F2b + 3 * ps, k, h2 * m1b);
-  ADD(a.ain + F_M2 * ps, k, m2b);
-  ADD(F1b + 2 * ps, k + 1, -h1 * m2b);
-  ADD(F1b + 2 * ps, k, h1 * m2b);
-  ADD(F2b + ps, k + pitch, -h2 * m2b);
-  ADD(F2b + ps, k, h2 * m2b);
+  double dhb = 0.0, m1b = 0.0, m2b = 0.0, b1cb = 0.0, b2cb = 0.0;
+  if (Eb != 0.0) {
+    const CorrCtx c = corr_ctx(a);
+    const double h1 = c.hdtodx1, h2 = c.hdtodx2;
+    const double* F1 = a.sc + S_F1 * ps;
+    const double* F2 = a.sc + S_F2 * ps;
+    const double dhalf = c.s(F_D, k) - h1 * (F1[k + 1] - F1[k]) - h2 * (F2[k + pitch] - F2[k]);
+    const double m1 = c.s(F_M1, k) - h1 * (F1[ps + k + 1] - F1[ps + k]) -
+                      h2 * (F2[3 * ps + k + pitch] - F2[3 * ps + k]);
+    const double m2 = c.s(F_M2, k) - h1 * (F1[2 * ps + k + 1] - F1[2 * ps + k]) -
+                      h2 * (F2[ps + k + pitch] - F2[ps + k]);
+    const double b1c = 0.5 * (c.b1h(k) + c.b1h(k + 1));
+    const double b2c = 0.5 * (c.b2h(k) + c.b2h(k + pitch));
+    const double emf = (b1c * m2 - b2c * m1) / dhalf;
+    b1cb = Eb * m2 / dhalf;
+    m2b = Eb * b1c / dhalf;
+    b2cb = -Eb * m1 / dhalf;
+    m1b = -Eb * b2c / dhalf;
+    dhb = -Eb * emf / dhalf;
+    a.ain[F_D * ps + k] += dhb;
+    a.ain[F_M1 * ps + k] += m1b;
+    a.ain[F_M2 * ps + k] += m2b;
+  }
+  double* ab = a.ab;
+  ab[A_HD * ps + k] = dhb;
+  ab[A_HM1 * ps + k] = m1b;
+  ab[A_HM2 * ps + k] = m2b;
+  ab[A_HB1 * ps + k] = b1cb;
+  ab[A_HB2 * ps + k] = b2cb;
 }
 
-// Steps 5, 6 (:284-393) and the half-step face fields of Step 4 (:263-275)
-// over [js-1,je+2] x [is-1,ie+2]
+// Steps 5, 6 (:284-393), the half-step face fields of Step 4 (:263-275) and
+// the flux part of Step 7, reversed as a gather: each thread owns one cell /
+// face / corner position c of [js-2,je+2] x [is-2,ie+2] and stores the
+// adjoints of the predictor fluxes and the predictor corner emf there, and
+// adds the state adjoints of c. Every adjoint read is zero outside the range
+// its writer covers, so no position needs a range test.
+//
+// The x1-face states at faces c+1 (left) and c (right) both take their 1/2 dt
+// source terms from cell c and the face-field difference B1i[c+1] - B1i[c]
+// (:296-330), and both are corrected with the x2 fluxes at c and c+pitch
+// (left at c+1: at c+1-1): with X1(c) = Lb(c+1) + Rb(c) the sources' adjoint
+// is linear in X1(c), and the x2 flux at c receives h2 (X1(c) - X1(c-pitch)).
+// Likewise for x2 faces with X2(c) = Lb(c+pitch) + Rb(c).
 __global__ void __launch_bounds__(kTileX* kTileY) ak_transverse(AdjArgs a) {
   const Inst& g = a.g;
   int j, i;
-  if (!range_ij(g.js - 1, g.je + 2, g.is - 1, g.ie + 2, j, i)) return;
+  if (!range_ij(g.js - 2, g.je + 2, g.is - 2, g.ie + 2, j, i)) return;
   const size_t ps = g.ps;
-  const int pitch = g.pitch;
-  const uint32_t k = (uint32_t)(j * pitch + i);
+  const uint32_t pitch = (uint32_t)g.pitch;
+  const uint32_t c = (uint32_t)j * pitch + (uint32_t)i;
   const double h1 = 0.5 * a.dt / g.dx1, h2 = 0.5 * a.dt / g.dx2;
   const double* st = a.st;
+  const double* ab = a.ab;
   double* ain = a.ain;
-  // MHD source terms of one face state with the cell-centred field of cell kc
-  // and the face-field difference B[kp] - B[km]; perm: components of the
-  // state receiving (b1, b2, b3) and the transverse-field slot
-  auto src_b = [&](const Cons7& Xb, uint32_t kc, int bplane, uint32_t kp, uint32_t km,
-                   double h, double xb1, double xb2, double xb3, double xbt) {
-    const double db = st[bplane * ps + kp] - st[bplane * ps + km];
-    const double b1 = st[F_B1C * ps + kc], b2 = st[F_B2C * ps + kc], b3 = st[F_B3C * ps + kc];
-    const double d = st[F_D * ps + kc], v3 = st[F_M3 * ps + kc] / d;
-    const double dbb = h * (xb1 * b1 + xb2 * b2 + xb3 * b3 + xbt * v3 + Xb.e * b3 * v3);
-    ADD(ain + bplane * ps, kp, dbb);
-    ADD(ain + bplane * ps, km, -dbb);
-    ADD(ain + F_B1C * ps, kc, h * db * xb1);
-    ADD(ain + F_B2C * ps, kc, h * db * xb2);
-    ADD(ain + F_B3C * ps, kc, h * db * (xb3 + Xb.e * v3));
-    const double v3b = h * db * (xbt + Xb.e * b3);
-    ADD(ain + F_M3 * ps, kc, v3b / d);
-    ADD(ain + F_D * ps, kc, -v3b * v3 / d);
+  // X1 and X2 components in Cons7 order (d, mx, my, mz, e, by, bz)
+  auto X1 = [&](uint32_t kk, int n) {
+    return ab[(A_UL1 + n) * ps + kk + 1] + ab[(A_UR1 + n) * ps + kk];
   };
-  if (i >= g.is && i <= g.ie + 1 && j <= g.je + 1) {  // corrected x1-face states
+  auto X2 = [&](uint32_t kk, int n) {
+    return ab[(A_UL2 + n) * ps + kk + pitch] + ab[(A_UR2 + n) * ps + kk];
+  };
+  // h_dir * (the source terms' adjoint on the face field), per cell
+  //   X.(b1, b2, b3, bt) components: x1 states (mx, my, mz, bz); x2 states
+  //   (d, M2, M3, M1, E, B3, B1) -> (mz, mx, my, by)
+  auto dbb = [&](uint32_t kk, double h, double xb1, double xb2, double xb3, double xbt,
+                 double xe) {
+    const double b1 = st[F_B1C * ps + kk], b2 = st[F_B2C * ps + kk], b3 = st[F_B3C * ps + kk];
+    const double v3 = st[F_M3 * ps + kk] / st[F_D * ps + kk];
+    return h * (xb1 * b1 + xb2 * b2 + xb3 * b3 + xbt * v3 + xe * b3 * v3);
+  };
+  auto dbb1 = [&](uint32_t kk) {
+    return dbb(kk, h1, X1(kk, 1), X1(kk, 2), X1(kk, 3), X1(kk, 6), X1(kk, 4));
+  };
+  auto dbb2 = [&](uint32_t kk) {
+    return dbb(kk, h2, X2(kk, 3), X2(kk, 1), X2(kk, 2), X2(kk, 5), X2(kk, 4));
+  };
+  // half-step face fields: Riemann part + Step 7's face averages
+  auto hb1 = [&](uint32_t kk) {
+    return ab[A_B1H * ps + kk] + 0.5 * (ab[A_HB1 * ps + kk] + ab[A_HB1 * ps + kk - 1]);
+  };
+  auto hb2 = [&](uint32_t kk) {
+    return ab[A_B2H * ps + kk] + 0.5 * (ab[A_HB2 * ps + kk] + ab[A_HB2 * ps + kk - pitch]);
+  };
+  const double* HD = ab + A_HD * ps;
+  const double* HM1 = ab + A_HM1 * ps;
+  const double* HM2 = ab + A_HM2 * ps;
+  // x2 predictor fluxes (d, M2, M3, M1, E, B3, B1): x1 states (d, mx, my, mz, e, bz) were
+  // corrected with components (0, 3, 1, 2, 4, 5); Step 7 used d, M1, M2 = (0, 3, 1)
+  {
     double* F2b = a.ab + A_F2 * ps;
-    const Cons7 Lb = ld7(a.ab + A_UL1 * ps, ps, k), Rb = ld7(a.ab + A_UR1 * ps, ps, k);
-    auto flux_b = [&](const Cons7& Xb, uint32_t f1, uint32_t f0) {
-      // X.d,mx,my,mz,e,bz -= h2 * (F2 d, mz, mx, my, e, by)[f1] - [f0]
-      const int comp[6] = {0, 3, 1, 2, 4, 5};
-      const double v[6] = {Xb.d, Xb.mx, Xb.my, Xb.mz, Xb.e, Xb.bz};
-      for (int n = 0; n < 6; n++) {
-        ADD(F2b + comp[n] * ps, f1, -h2 * v[n]);
-        ADD(F2b + comp[n] * ps, f0, h2 * v[n]);
-      }
-    };
-    flux_b(Lb, k + pitch - 1, k - 1);
-    flux_b(Rb, k + pitch, k);
-    src_b(Lb, k - 1, F_B1I, k, k - 1, h1, Lb.mx, Lb.my, Lb.mz, Lb.bz);
-    src_b(Rb, k, F_B1I, k + 1, k, h1, Rb.mx, Rb.my, Rb.mz, Rb.bz);
+    const int src[7] = {0, 2, 3, 1, 4, 6, -1};  // x1-state component feeding F2 component
+    for (int n = 0; n < 7; n++) {
+      double v = (src[n] < 0) ? 0.0 : h2 * (X1(c, src[n]) - X1(c - pitch, src[n]));
+      if (n == 0) v += h2 * (HD[c] - HD[c - pitch]);
+      if (n == 3) v += h2 * (HM1[c] - HM1[c - pitch]);
+      if (n == 1) v += h2 * (HM2[c] - HM2[c - pitch]);
+      F2b[n * ps + c] = v;
+    }
   }
-  if (j >= g.js && j <= g.je + 1 && i <= g.ie + 1) {  // corrected x2-face states
+  // x1 predictor fluxes (d, M1, M2, M3, E, B2, B3): x2 states (d, mx, my, mz, e, by) were
+  // corrected with components (0, 2, 3, 1, 4, 6); Step 7 used d, M1, M2 = (0, 1, 2)
+  {
     double* F1b = a.ab + A_F1 * ps;
-    const Cons7 Lb = ld7(a.ab + A_UL2 * ps, ps, k), Rb = ld7(a.ab + A_UR2 * ps, ps, k);
-    auto flux_b = [&](const Cons7& Xb, uint32_t f1, uint32_t f0) {
-      // X.d,mx,my,mz,e,by -= h1 * (F1 d, my, mz, mx, e, bz)[f1] - [f0]
-      const int comp[6] = {0, 2, 3, 1, 4, 6};
-      const double v[6] = {Xb.d, Xb.mx, Xb.my, Xb.mz, Xb.e, Xb.by};
-      for (int n = 0; n < 6; n++) {
-        ADD(F1b + comp[n] * ps, f1, -h1 * v[n]);
-        ADD(F1b + comp[n] * ps, f0, h1 * v[n]);
-      }
-    };
-    flux_b(Lb, k - pitch + 1, k - pitch);
-    flux_b(Rb, k + 1, k);
-    // x2 states are (d, M2, M3, M1, E, B3, B1): b1 enters M1 = mz, b2 enters
-    // M2 = mx, b3 enters M3 = my, v3 enters B3 = by
-    src_b(Lb, k - pitch, F_B2I, k, k - pitch, h2, Lb.mz, Lb.mx, Lb.my, Lb.by);
-    src_b(Rb, k, F_B2I, k + pitch, k, h2, Rb.mz, Rb.mx, Rb.my, Rb.by);
-  }
-  if (j <= g.je + 1) {  // B1h = B1i - h2 (emf[j+1] - emf[j]) on [js-1,je+1] x [is-1,ie+2]
-    const double hb = a.ab[A_B1H * ps + k];
-    if (hb != 0.0) {
-      ADD(ain + F_B1I * ps, k, hb);
-      ADD(a.ab + A_EMFP * ps, k + pitch, -h2 * hb);
-      ADD(a.ab + A_EMFP * ps, k, h2 * hb);
+    const int src[7] = {0, 3, 1, 2, 4, -1, 5};  // x2-state component feeding F1 component
+    for (int n = 0; n < 7; n++) {
+      double v = (src[n] < 0) ? 0.0 : h1 * (X2(c, src[n]) - X2(c - 1, src[n]));
+      if (n == 0) v += h1 * (HD[c] - HD[c - 1]);
+      if (n == 1) v += h1 * (HM1[c] - HM1[c - 1]);
+      if (n == 2) v += h1 * (HM2[c] - HM2[c - 1]);
+      F1b[n * ps + c] = v;
     }
   }
-  if (i <= g.ie + 1) {  // B2h = B2i + h1 (emf[i+1] - emf[i]) on [js-1,je+2] x [is-1,ie+1]
-    const double hb = a.ab[A_B2H * ps + k];
-    if (hb != 0.0) {
-      ADD(ain + F_B2I * ps, k, hb);
-      ADD(a.ab + A_EMFP * ps, k + 1, h1 * hb);
-      ADD(a.ab + A_EMFP * ps, k, -h1 * hb);
-    }
+  // predictor corner emf from the half-step face fields (:266-275)
+  a.ab[A_EMFP * ps + c] = h2 * (hb1(c) - hb1(c - pitch)) + h1 * (hb2(c - 1) - hb2(c));
+  // face fields: half-step fields and the source terms' face-field differences
+  ain[F_B1I * ps + c] += hb1(c) + dbb1(c - 1) - dbb1(c);
+  ain[F_B2I * ps + c] += hb2(c) + dbb2(c - pitch) - dbb2(c);
+  // the source terms' cell-centred B and v3 = M3 / d of cell c
+  {
+    const double b3 = st[F_B3C * ps + c];
+    const double d = st[F_D * ps + c], v3 = st[F_M3 * ps + c] / d;
+    const double db1 = h1 * (st[F_B1I * ps + c + 1] - st[F_B1I * ps + c]);
+    const double db2 = h2 * (st[F_B2I * ps + c + pitch] - st[F_B2I * ps + c]);
+    const double x1e = X1(c, 4), x2e = X2(c, 4);
+    ain[F_B1C * ps + c] += db1 * X1(c, 1) + db2 * X2(c, 3);
+    ain[F_B2C * ps + c] += db1 * X1(c, 2) + db2 * X2(c, 1);
+    ain[F_B3C * ps + c] += db1 * (X1(c, 3) + x1e * v3) + db2 * (X2(c, 2) + x2e * v3);
+    const double v3b = db1 * (X1(c, 6) + x1e * b3) + db2 * (X2(c, 5) + x2e * b3);
+    ain[F_M3 * ps + c] += v3b / d;
+    ain[F_D * ps + c] += -v3b * v3 / d;
   }
 }
 
@@ -632,7 +643,7 @@ extern "C" hipError_t athmi_adjoint_launch(const AdjArgs* ap, int kernel, hipStr
       hipLaunchKernelGGL(ak_half, grid2(g.js - 1, g.je + 1, g.is - 1, g.ie + 1), b2, 0, s, a);
       break;
     case AK_TRANSVERSE:
-      hipLaunchKernelGGL(ak_transverse, grid2(g.js - 1, g.je + 2, g.is - 1, g.ie + 2), b2, 0, s, a);
+      hipLaunchKernelGGL(ak_transverse, grid2(g.js - 2, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);
       break;
     case AK_CORNER_P:
       hipLaunchKernelGGL(ak_corner<true>, grid2(g.js - 1, g.je + 2, g.is - 1, g.ie + 2), b2, 0, s, a);

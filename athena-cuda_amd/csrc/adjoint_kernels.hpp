@@ -13,14 +13,14 @@ namespace athmi {
 // is stored by one kernel over the range its readers cover (entries outside
 // that range stay zero from the clear at the start of a reverse pass).
 enum AdjPlane {
-  A_F1 = 0,     // predictor x1 fluxes
-  A_F2 = 7,     // predictor x2 fluxes
-  A_B1H = 14,   // half-step face fields
-  A_B2H = 15,
-  A_EMFP = 16,  // corner emf, predictor
-  A_ECH = 17,   // cell-centred emf3 at t^{n+1/2}
-  A_ECC = 18,   // cell-centred emf3 at t^n
-  NACC = 19,
+  A_ECH = 0,    // cell-centred emf3 at t^{n+1/2}
+  A_ECC = 1,    // cell-centred emf3 at t^n
+  NACC = 2,
+  A_F1 = 2,     // predictor x1 fluxes
+  A_F2 = 9,     // predictor x2 fluxes
+  A_B1H = 16,   // half-step face fields, from the corrector Riemann fluxes
+  A_B2H = 17,
+  A_EMFP = 18,  // corner emf, predictor
   A_F1C = 19,   // corrector x1 fluxes
   A_F2C = 26,   // corrector x2 fluxes
   A_UL1 = 33,   // x1-face states (predictor value = corrected value's identity part)
@@ -32,7 +32,12 @@ enum AdjPlane {
   A_WL2 = 75,   // ... at x2 faces
   A_WR2 = 82,
   A_EMFC = 89,  // corner emf, corrector
-  NADJ = 90
+  A_HD = 90,    // Step 7 per cell: half-step d, M1, M2
+  A_HM1 = 91,
+  A_HM2 = 92,
+  A_HB1 = 93,   // ... and the cell-centred half-step B1c, B2c
+  A_HB2 = 94,
+  NADJ = 95
 };
 
 // Everything the adjoint kernels of one step need
