@@ -55,6 +55,7 @@ SIGNATURES = {
     "athmi_set_clock": (_I, [_P, _I, _D, _D, _I]),
     "athmi_get_clock": (_I, [_P, _I, _DP, _DP, _IP]),
     "athmi_set_limits": (_I, [_P, _I, _D, _I]),
+    "athmi_set_pause": (_I, [_P, _I, _D]),
     "athmi_set_bvals": (_I, [_P]),
     "athmi_new_dt": (_I, [_P]),
     "athmi_integrate": (_I, [_P]),
@@ -218,6 +219,10 @@ class Grid:
 
     def set_limits(self, tlim, nlim=-1, inst=-1):
         _check(lib().athmi_set_limits(self._h, inst, tlim, nlim), "set_limits")
+
+    def set_pause(self, t_next, inst=-1):
+        """the main-loop step waits once time >= t_next (an output is due)"""
+        _check(lib().athmi_set_pause(self._h, inst, t_next), "set_pause")
 
     # --- the reference's routines -------------------------------------------
     def set_bvals(self):

@@ -110,6 +110,7 @@ int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
     std::memset(&hc[k], 0, sizeof(Clock));
     hc[k].cour_no = desc[k].cour_no;
     hc[k].tlim = HUGE_VAL;
+    hc[k].tpause = HUGE_VAL;
     hc[k].nlim = -1;
     Inst& I = g->hinst[k];
     I.state[0] = g->dmem + off[k];
@@ -140,6 +141,7 @@ int athmi_destroy(athmi_grid* g) {
   if (g->dmem) (void)hipFree(g->dmem);
   if (g->dclk) (void)hipFree(g->dclk);
   if (g->dinst) (void)hipFree(g->dinst);
+  if (g->outbuf) (void)hipFree(g->outbuf);
   if (g->own_stream && g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
   return ATHMI_OK;
@@ -247,6 +249,20 @@ int athmi_set_limits(athmi_grid* g, int inst, double tlim, int nlim) {
   return ATHMI_OK;
 }
 
+int athmi_set_pause(athmi_grid* g, int inst, double t_next) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  const int lo = inst < 0 ? 0 : inst, hi = inst < 0 ? g->n - 1 : inst;
+  if (inst >= g->n) return fail(ATHMI_EINVAL, "instance index out of range");
+  for (int k = lo; k <= hi; k++) {
+    Clock c;
+    int rc = g->read_clock(k, c);
+    if (rc) return rc;
+    c.tpause = t_next;
+    if ((rc = g->write_clock_head(k, c))) return rc;
+  }
+  return ATHMI_OK;
+}
+
 int athmi_set_bvals(athmi_grid* g) {
   if (!g) return fail(ATHMI_EINVAL, "null handle");
   HIP_TRY(g->launch(&g->bl, SK_GHOST_CUR, g->stream));
@@ -303,7 +319,7 @@ int athmi_run(athmi_grid* g, int max_steps, int* steps) {
                         offsetof(Clock, steps_run), g->n, hipMemcpyDeviceToHost));
     bool running = false;
     for (const Clock& c : hc)
-      if (c.time < c.tlim && (c.nlim < 0 || c.nstep < c.nlim)) running = true;
+      if (c.time < c.tlim && (c.nlim < 0 || c.nstep < c.nlim) && c.time < c.tpause) running = true;
     if (!running) break;
     int m = chunk;
     if (max_steps >= 0) m = std::min(m, max_steps - issued);
@@ -494,28 +510,6 @@ int athmi_device_state(athmi_grid* g, int inst, double** planes, int* pitch) {
 }
 
 // ---- problem generators ------------------------------------------------------
-static HostGeom geom_of(const athmi_grid* g, int inst) {
-  const athmi_desc& d = g->desc[inst];
-  const Inst& I = g->hinst[inst];
-  HostGeom G;
-  G.Nx1 = d.nx1;
-  G.Nx2 = d.nx2;
-  G.is = I.is;
-  G.ie = I.ie;
-  G.js = I.js;
-  G.je = I.je;
-  G.sx = I.sx;
-  G.sy = I.sy;
-  G.dx1 = I.dx1;
-  G.dx2 = I.dx2;
-  G.x1_0 = d.x1min;
-  G.x2_0 = d.x2min;
-  G.idisp = -I.is;
-  G.jdisp = -I.js;
-  G.gamma_1 = I.g1;
-  return G;
-}
-
 static int upload_fresh(athmi_grid* g, int inst, const HostState& S) {
   return athmi_upload(g, inst, S.gas.data(), S.b1i.data(), S.b2i.data(), S.b3i.data());
 }

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "athmi.h"
+#include "host_problems.hpp"
 #include "step_kernels.hpp"
 
 ATHMI_DECLARE_VARIANT(exact)
@@ -70,6 +71,8 @@ struct athmi_grid {
   hipGraphExec_t graph = nullptr;
   std::vector<std::vector<double>> cpaw_soln;  // initial cpaw2d states (8 planes)
   std::vector<double> stage;                   // host SoA staging
+  void* outbuf = nullptr;                      // device staging of the output formats
+  size_t outbuf_bytes = 0;
 
   void drop_graph() {
     if (graph) (void)hipGraphExecDestroy(graph);
@@ -100,6 +103,29 @@ struct athmi_grid {
     return ATHMI_OK;
   }
 };
+
+// the reference's Grid geometry of one instance (init_grid.c:25-118)
+inline HostGeom geom_of(const athmi_grid* g, int inst) {
+  const athmi_desc& d = g->desc[inst];
+  const Inst& I = g->hinst[inst];
+  HostGeom G;
+  G.Nx1 = d.nx1;
+  G.Nx2 = d.nx2;
+  G.is = I.is;
+  G.ie = I.ie;
+  G.js = I.js;
+  G.je = I.je;
+  G.sx = I.sx;
+  G.sy = I.sy;
+  G.dx1 = I.dx1;
+  G.dx2 = I.dx2;
+  G.x1_0 = d.x1min;
+  G.x2_0 = d.x2min;
+  G.idisp = -I.is;
+  G.jdisp = -I.js;
+  G.gamma_1 = I.g1;
+  return G;
+}
 
 inline int issue_seq(athmi_grid* g, const int* seq, size_t n) {
   for (size_t k = 0; k < n; k++) HIP_TRY(g->launch(&g->bl, seq[k], g->stream));

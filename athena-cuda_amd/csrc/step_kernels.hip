@@ -631,7 +631,10 @@ __global__ void __launch_bounds__(kGhostBlock) k_ghost(const Inst* __restrict__ 
 // ---------------------------------------------------------------------------
 // Clock kernels: one workgroup of 256 threads per instance.
 // mode 0: main-loop step (main.c:221-232): run while time < tlim and
-//         (nlim < 0 or nstep < nlim); clamp dt to reach tlim exactly.
+//         (nlim < 0 or nstep < nlim); clamp dt to reach tlim exactly. The
+//         step also waits (does not run) once time >= tpause, so that the
+//         host can write the outputs due at the top of the iteration
+//         (main.c:226, output.c:372-376) before it runs.
 // mode 1: unconditional single integration (drop-in integrate_2d).
 __global__ void __launch_bounds__(256) k_begin(const Inst* __restrict__ insts, int mode) {
   Clock* c = insts[blockIdx.x].clk;
@@ -639,7 +642,7 @@ __global__ void __launch_bounds__(256) k_begin(const Inst* __restrict__ insts, i
   if (threadIdx.x == 0) {
     int act = 1;
     if (mode == 0) {
-      act = (c->time < c->tlim) && (c->nlim < 0 || c->nstep < c->nlim);
+      act = (c->time < c->tlim) && (c->nlim < 0 || c->nstep < c->nlim) && (c->time < c->tpause);
       if (act && (c->tlim - c->time) < c->dt) c->dt = (c->tlim - c->time);
     }
     c->active = act;

@@ -38,6 +38,7 @@ constexpr int kDtSlots = 256;  // partial max slots of the fused dt reduction
 // plus the main-loop limits of main.c:217-218).
 struct Clock {
   double dt, time, tlim, cour_no;
+  double tpause;  // the main-loop step does not run once time >= tpause (an output is due)
   int nstep, nlim, active, cur;  // cur: which ping-pong state buffer holds the state
   int steps_run, pad0, pad1, pad2;
   unsigned long long divb_bits;           // max |div B| (bit pattern), diagnostics

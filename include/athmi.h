@@ -80,6 +80,12 @@ int athmi_set_clock(athmi_grid *g, int inst, double dt, double time, int nstep);
 int athmi_get_clock(athmi_grid *g, int inst, double *dt, double *time, int *nstep);
 /* main-loop limits (main.c:217-218: tlim, nlim; nlim < 0 = none); inst = -1: all */
 int athmi_set_limits(athmi_grid *g, int inst, double tlim, int nlim);
+/* Output cadence (main.c:226 calls data_output, output.c:362-376, at the top
+ * of every iteration): the main-loop step of an instance does not run once
+ * its time >= t_next, so that the caller can write the outputs due
+ * (athmi_output.h) before the step; HUGE_VAL (the default) never waits.
+ * inst = -1: all instances. */
+int athmi_set_pause(athmi_grid *g, int inst, double t_next);
 
 /* set_bvals(Grid*, 0) (set_bvals.c:114) on every instance */
 int athmi_set_bvals(athmi_grid *g);

@@ -24,10 +24,15 @@ void problem_blast(Grid *pG);
 void problem_field_loop(Grid *pG);
 void problem_cpaw2d(Grid *pG);
 void userwork_after_loop_cpaw2d(Grid *pG);
+Gasfun_t gue_blast(const char *expr);
+Gasfun_t gue_field_loop(const char *expr);
+Gasfun_t gue_cpaw2d(const char *expr);
 
 static Grid G;
 static int have_grid = 0;
 static int par_is_open = 0;
+static int cur_problem = -1;
+static char outname[256] = "ref";
 
 /* main.c:64-94: parameters, Courant number and EOS constants */
 int ref_open(const char *athinput, int nover, const char **over)
@@ -61,7 +66,8 @@ void ref_set_gamma(double gamma)
 int ref_setup(int problem)
 {
   memset(&G, 0, sizeof(G));
-  G.outfilename = "ref";
+  G.outfilename = outname;
+  cur_problem = problem;
   init_grid(&G);
   if (problem == 0) problem_blast(&G);
   else if (problem == 1) problem_field_loop(&G);
@@ -102,6 +108,49 @@ int ref_run(double tlim, int nlim)
 {
   int n = 0;
   while (G.time < tlim && (nlim < 0 || n < nlim)) {
+    ref_step(tlim);
+    n++;
+  }
+  return n;
+}
+
+/* The reference's output layer (output.c, dump_binary.c, output_ppm.c,
+ * dump_tab.c, ath_files.c) on the harness grid. Files go to the current
+ * directory as <outfilename>[.num][.id].<ext>. */
+Gasfun_t get_usr_expr(const char *expr)  /* the linked problem's (output.c:233) */
+{
+  if (cur_problem == 0) return gue_blast(expr);
+  if (cur_problem == 1) return gue_field_loop(expr);
+  if (cur_problem == 2) return gue_cpaw2d(expr);
+  return NULL;
+}
+
+void ref_set_outfilename(const char *name)  /* main.c:90 (job/problem_id) */
+{
+  strncpy(outname, name, sizeof(outname) - 1);
+  outname[sizeof(outname) - 1] = 0;
+}
+
+/* getRGB (output.c:862-880); name must be one of the palettes of palette.h */
+float *getRGB(char *name);
+void ref_get_palette(const char *name, float *out768)
+{
+  const float *rgb = getRGB((char *)name);
+  memcpy(out768, rgb, 768 * sizeof(float));
+}
+
+void ref_init_output(void) { init_output(&G); }
+void ref_data_output(int flag) { data_output(&G, NULL, flag); }
+void ref_output_destruct(void) { data_output_destruct(); }
+
+/* main.c:221-318 (ONLY_HOST) with data_output at the top of each iteration
+ * (main.c:226); returns the number of steps taken */
+int ref_run_with_output(double tlim, int nlim)
+{
+  int n = 0;
+  if (!have_grid) return -1;
+  while (G.time < tlim && (nlim < 0 || G.nstep < nlim)) {
+    data_output(&G, NULL, 0);
     ref_step(tlim);
     n++;
   }

@@ -17,7 +17,8 @@ def declared(header):
                   {"if", "while", "for", "sizeof", "return"})
 
 
-@pytest.mark.parametrize("header", ["athmi.h", "athena_dropin.h"])
+@pytest.mark.parametrize("header", ["athmi.h", "athena_dropin.h", "athmi_adjoint.h",
+                                    "athmi_output.h"])
 def test_library_exports_every_declared_entry_point(header):
     pkg = load_pkg()
     lib = ctypes.CDLL(pkg.LIB_PATH)
