@@ -73,6 +73,7 @@ SIGNATURES = {
     "athmi_pgen_field_loop": (_I, [_P, _I, _D, _D, _D, _I]),
     "athmi_pgen_cpaw2d": (_I, [_P, _I, _D, _D, _D, _D, _D, _DP]),
     "athmi_cpaw2d_l1": (_I, [_P, _I, _DP]),
+    "athmi_pgen_device": (_I, [_P, _I, _I, _P, _DP]),
     "athmi_last_error": (ctypes.c_char_p, []),
 }
 
@@ -155,6 +156,10 @@ def desc_from_athinput(par, flux="roe", variant="exact"):
                 gamma=_atof(par["problem"]["gamma"]), cour_no=_atof(par["time"]["cour_no"]),
                 bc=(_atoi(g["ibc_x1"]), _atoi(g["obc_x1"]), _atoi(g["ibc_x2"]), _atoi(g["obc_x2"])),
                 flux=flux, variant=variant)
+
+
+class _PgenParams(ctypes.Structure):
+    _fields_ = [("problem", ctypes.c_int), ("p", ctypes.c_double * 7)]
 
 
 class Grid:
@@ -292,6 +297,21 @@ class Grid:
         _check(lib().athmi_pgen_cpaw2d(self._h, inst, angle, b_par, b_perp, pres, v_par,
                                        ctypes.byref(lam)), "pgen_cpaw2d")
         return lam.value
+
+    def pgen_device(self, first, problems):
+        """the problem generators on the device for instances first,
+        first+1, ...: problems is a list of (name, params) with name in
+        PROBLEMS and params in the order of the host generators. Returns the
+        cpaw2d wavelengths (0 for the other problems)."""
+        n = len(problems)
+        arr = (_PgenParams * n)()
+        for m, (name, prm) in enumerate(problems):
+            arr[m].problem = PROBLEMS.index(name)
+            for q, v in enumerate(prm):
+                arr[m].p[q] = float(v)
+        lam = (ctypes.c_double * n)()
+        _check(lib().athmi_pgen_device(self._h, first, n, arr, lam), "pgen_device")
+        return list(lam)
 
     def cpaw2d_l1(self, inst=0):
         out = (ctypes.c_double * 9)()

@@ -44,7 +44,7 @@ def build(verbose_resources=False):
             run(cmd)
             objs.append(o)
         for src in ("athmi_capi.cpp", "calib.hip", "adjoint_host.cpp", "adjoint_capi.cpp",
-                    "adjoint_kernels.hip", "output.hip"):
+                    "adjoint_kernels.hip", "output.hip", "pgen.hip"):
             o = os.path.join(tmp, src.split(".")[0] + ".o")
             run([HIPCC, ARCH, *COMMON, *NO_PK, "-ffp-contract=off", "-c",
                  os.path.join(CSRC, src), "-o", o])

@@ -100,20 +100,17 @@ void problem_field_loop(const HostGeom& G, HostState& S, double rad, double amp,
 }
 
 namespace {
-struct Cpaw { double sin_a, cos_a, b_par, b_perp, k_par; };
 // cpaw2d.c:311-315: B_x = -dA3/dy, B_y = dA3/dx
-double A3(const Cpaw& c, double x1, double x2) {
+double A3(const CpawConst& c, double x1, double x2) {
   return c.b_par * (x1 * c.sin_a - x2 * c.cos_a) -
          (c.b_perp / c.k_par) * std::cos(c.k_par * (x1 * c.cos_a + x2 * c.sin_a));
 }
 }  // namespace
 
-// cpaw2d.c:56-185; one wavelength per direction, wave vector at
-// atan(Lx/Ly) to x1 unless angle is 0 or 90. soln receives the initial
-// conserved state (8 planes [sy][sx], Gas order) for cpaw2d_errors.
-double problem_cpaw2d(const HostGeom& G, HostState& S, double angle, double b_par, double b_perp,
-                      double pres, double v_par, std::vector<double>* soln) {
-  Cpaw c;
+// one wavelength per direction, wave vector at atan(Lx/Ly) to x1 unless
+// angle is 0 or 90
+CpawConst cpaw2d_constants(const HostGeom& G, double angle, double b_par, double b_perp) {
+  CpawConst c;
   double lambda;
   if (angle == 0.0) {
     c.sin_a = 0.0;
@@ -134,9 +131,19 @@ double problem_cpaw2d(const HostGeom& G, HostState& S, double angle, double b_pa
     if (c.cos_a >= c.sin_a) lambda = G.Nx1 * G.dx1 * c.cos_a;
     else lambda = G.Nx2 * G.dx2 * c.sin_a;
   }
+  c.lambda = lambda;
   c.k_par = 2.0 * kPi / lambda;
   c.b_par = b_par;
   c.b_perp = b_perp;
+  return c;
+}
+
+// cpaw2d.c:56-185. soln receives the initial conserved state (8 planes
+// [sy][sx], Gas order) for cpaw2d_errors.
+double problem_cpaw2d(const HostGeom& G, HostState& S, double angle, double b_par, double b_perp,
+                      double pres, double v_par, std::vector<double>* soln) {
+  const CpawConst c = cpaw2d_constants(G, angle, b_par, b_perp);
+  const double lambda = c.lambda;
   const double den = 1.0;
   const double v_perp = b_perp / std::sqrt(den);
   const double dx1 = G.dx1, dx2 = G.dx2;

@@ -35,6 +35,11 @@ void problem_blast(const HostGeom& G, HostState& S, double rin, double pa, doubl
                    double b0, double angle_deg, double cx, double cy);
 void problem_field_loop(const HostGeom& G, HostState& S, double rad, double amp, double vflow,
                         int iprob);
+// cpaw2d.c:67-104: the wave's direction and wavenumber for a grid (host libm)
+struct CpawConst {
+  double sin_a, cos_a, b_par, b_perp, k_par, lambda;
+};
+CpawConst cpaw2d_constants(const HostGeom& G, double angle, double b_par, double b_perp);
 double problem_cpaw2d(const HostGeom& G, HostState& S, double angle, double b_par, double b_perp,
                       double pres, double v_par, std::vector<double>* soln);
 void cpaw2d_errors(const HostGeom& G, const HostState& S, const std::vector<double>& soln,

@@ -57,12 +57,18 @@ def cells(params):
     return sum(p["nx1"] * p["nx2"] for p in params)
 
 
-def build(params, nlim, variant="exact"):
+def build(params, nlim, variant="exact", device_ics=False):
     """One handle holding every instance, initialised as main.c:121-151
-    (problem generator, set_bvals, first dt) with main-loop limit nlim."""
+    (problem generator, set_bvals, first dt) with main-loop limit nlim.
+    device_ics: generate the blasts on the device in one launch (the same
+    states as the host generator)."""
     g = Grid(params, variant=variant)
-    for k, p in enumerate(params):
-        g.pgen_blast(k, p["radius"], PAMB, p["prat"], p["b0"], p["angle"], p["cx"], p["cy"])
+    if device_ics:
+        g.pgen_device(0, [("blast", (p["radius"], PAMB, p["prat"], p["b0"], p["angle"], p["cx"],
+                                     p["cy"])) for p in params])
+    else:
+        for k, p in enumerate(params):
+            g.pgen_blast(k, p["radius"], PAMB, p["prat"], p["b0"], p["angle"], p["cx"], p["cy"])
     g.set_bvals()
     g.new_dt()
     g.set_limits(1e30, nlim)

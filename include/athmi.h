@@ -149,6 +149,23 @@ int athmi_pgen_field_loop(athmi_grid *g, int inst, double rad, double amp, doubl
                           int iprob);
 int athmi_pgen_cpaw2d(athmi_grid *g, int inst, double angle, double b_par, double b_perp,
                       double pres, double v_par, double *lambda);
+/* The same generators on the device, for instances [first, first + n) in one
+ * launch over their cells (the batch's initial states never cross PCIe).
+ * params[m] holds the problem and its <problem> parameters in the order of the
+ * host functions above: blast (radius, pamb, prat, b0, angle, cx, cy),
+ * field_loop (rad, amp, vflow, iprob), cpaw2d (angle, b_par, b_perp, pres,
+ * v_par); lambda[m] (may be NULL) receives cpaw2d's wavelength. Per-grid
+ * constants use the host libm: blast and field_loop states are identical to
+ * the host generators'; cpaw2d's per-cell sin/cos use the device libm (within
+ * an ulp). The host generators stay the parity mode. */
+enum { ATHMI_PGEN_BLAST = 0, ATHMI_PGEN_FIELD_LOOP = 1, ATHMI_PGEN_CPAW2D = 2 };
+typedef struct athmi_pgen_params {
+  int problem;
+  double p[7];
+} athmi_pgen_params;
+int athmi_pgen_device(athmi_grid *g, int first, int n, const athmi_pgen_params *params,
+                      double *lambda);
+
 /* cpaw2d.c:222-302: out[0] = RMS error, out[1..8] = L1 error per variable in
  * Gas order (d, M1, M2, M3, E, B1c, B2c, B3c), each divided by Nx1*Nx2 */
 int athmi_cpaw2d_l1(athmi_grid *g, int inst, double *out);
