@@ -6,7 +6,9 @@
 // output state and forward scratch (predictor face states, conserved and
 // primitive, and fluxes, predictor corner emf, corrector fluxes) into the
 // segment's per-step slots; then each step is reversed by the adjoint
-// kernels, last step first.
+// kernels, last step first. With two banks of slots (when they fit in device
+// memory) the re-run of segment s-1 on the handle's stream overlaps the
+// reversal of segment s on a second stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,20 +44,28 @@ struct athmi_adjoint {
   athmi_grid* g = nullptr;
   double dt = 0.0;
   int nsteps = 0, every = 0, nck = 0;
+  int nbank = 1;            // banks of segment slots (2: re-run and reversal overlap)
   int cur = 0;              // host copy of the clock's current buffer
   bool have_forward = false;
   double* mem = nullptr;    // checkpoints | segment states | adjoint state x2 | adjoint planes
   double* ck = nullptr;
-  double* seg = nullptr;     // K + 1 states: the input of each step of a segment
+  double* seg = nullptr;     // per bank K + 1 states: the input of each step of a segment
   double* adj[2] = {nullptr, nullptr};
   double* ab = nullptr;
-  double* segsc = nullptr;   // K slots of kSlotPlanes: forward scratch of each step
+  double* segsc = nullptr;   // per bank K slots of kSlotPlanes: forward scratch of each step
   size_t state_doubles = 0;  // NSTATE * ps
   size_t slot_doubles = 0;   // kSlotPlanes * ps
-  // 2K device descriptors of the re-run: step m of a segment whose first step
-  // reads state buffer p is dslot[p * K + m] (state buffers seg[m], seg[m+1],
-  // scratch slot m)
+  // device descriptors of the re-run: step m of a segment in bank b whose
+  // first step reads state buffer p is dslot[(2 b + p) K + m] (state buffers
+  // seg[b][m], seg[b][m+1], scratch slot [b][m])
   Inst* dslot = nullptr;
+  hipStream_t astream = nullptr;           // the reversal's stream
+  hipEvent_t ev_rerun[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr}, ev_sync = nullptr;
+
+  double* seg_state(int b, int m) const {
+    return seg + ((size_t)b * (every + 1) + m) * state_doubles;
+  }
+  double* slot(int b, int m) const { return segsc + ((size_t)b * every + m) * slot_doubles; }
 };
 
 // launch a kernel sequence of the instance through the descriptor ins
@@ -88,20 +98,44 @@ static int copy_state(athmi_adjoint* h, double* dst, const double* src) {
 }
 
 // dense [NSTATE][Nx2+8][Nx1+8] <-> pitched planes
-static int dense_to_planes(athmi_adjoint* h, double* planes, const double* dense) {
+static int dense_to_planes(athmi_adjoint* h, double* planes, const double* dense, hipStream_t s) {
   const Inst& I = h->g->hinst[0];
   for (int q = 0; q < NSTATE; q++)
     HIP_TRY(hipMemcpy2DAsync(planes + q * I.ps, I.pitch * sizeof(double),
                              dense + (size_t)q * I.sx * I.sy, I.sx * sizeof(double),
-                             I.sx * sizeof(double), I.sy, hipMemcpyDeviceToDevice, h->g->stream));
+                             I.sx * sizeof(double), I.sy, hipMemcpyDeviceToDevice, s));
   return ATHMI_OK;
 }
-static int planes_to_dense(athmi_adjoint* h, double* dense, const double* planes) {
+static int planes_to_dense(athmi_adjoint* h, double* dense, const double* planes, hipStream_t s) {
   const Inst& I = h->g->hinst[0];
   for (int q = 0; q < NSTATE; q++)
     HIP_TRY(hipMemcpy2DAsync(dense + (size_t)q * I.sx * I.sy, I.sx * sizeof(double),
                              planes + q * I.ps, I.pitch * sizeof(double), I.sx * sizeof(double),
-                             I.sy, hipMemcpyDeviceToDevice, h->g->stream));
+                             I.sy, hipMemcpyDeviceToDevice, s));
+  return ATHMI_OK;
+}
+
+// re-run segment s (steps n0..n1-1) from its checkpoint into bank b on the
+// handle's stream
+static int rerun_segment(athmi_adjoint* h, int s, int b) {
+  const int n0 = s * h->every, n1 = std::min(n0 + h->every, h->nsteps);
+  const size_t ps = h->g->hinst[0].ps;
+  const Inst* slots = h->dslot + (size_t)(2 * b + h->cur) * h->every;
+  int rc;
+  if ((rc = copy_state(h, h->seg_state(b, 0), h->ck + (size_t)s * h->state_doubles))) return rc;
+  for (int m = n0; m < n1; m++) {
+    const Inst* ins = slots + (m - n0);
+    double* sc = h->slot(b, m - n0);
+    if ((rc = issue_on(h, ins, kSeqForwardScratch, sizeof(kSeqForwardScratch) / sizeof(int))))
+      return rc;
+    HIP_TRY(hipMemcpyAsync(sc + kSlotEmf * ps, sc + S_EMF * ps, ps * sizeof(double),
+                           hipMemcpyDeviceToDevice, h->g->stream));
+    if (m + 1 < n1) {
+      if ((rc = issue_on(h, ins, kSeqFinishStep, sizeof(kSeqFinishStep) / sizeof(int)))) return rc;
+      h->cur ^= 1;  // k_end swaps the buffers: the output is seg[b][m + 1]
+      if ((rc = issue_on(h, ins, kSeqGhostCur, 1))) return rc;
+    }
+  }
   return ATHMI_OK;
 }
 
@@ -130,29 +164,47 @@ int athmi_adjoint_create(const athmi_desc* desc, double dt, int nsteps, int ever
   const Inst& I = h->g->hinst[0];
   h->state_doubles = (size_t)NSTATE * I.ps;
   h->slot_doubles = (size_t)kSlotPlanes * I.ps;
-  const size_t total = h->state_doubles * (h->nck + h->every + 3) + (size_t)NADJ * I.ps +
-                       h->slot_doubles * h->every;
-  if (hipMalloc(&h->mem, total * sizeof(double)) != hipSuccess ||
-      hipMalloc(&h->dslot, sizeof(Inst) * 2 * h->every) != hipSuccess) {
+  // two banks of segment slots when they leave a fifth of the free memory
+  auto total_for = [&](int nb) {
+    return h->state_doubles * (h->nck + (size_t)nb * (h->every + 1) + 2) + (size_t)NADJ * I.ps +
+           h->slot_doubles * nb * h->every;
+  };
+  size_t free_b = 0, total_b = 0;
+  h->nbank = (hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+              total_for(2) * sizeof(double) < free_b / 5 * 4) ? 2 : 1;
+  if (hipMalloc(&h->mem, total_for(h->nbank) * sizeof(double)) != hipSuccess ||
+      hipMalloc(&h->dslot, sizeof(Inst) * 4 * h->every) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->astream, hipStreamNonBlocking) != hipSuccess) {
     athmi_adjoint_destroy(h);
     return fail(ATHMI_ENOMEM, "athmi_adjoint_create: no room for checkpoints");
   }
+  for (int b = 0; b < 2; b++)
+    if (hipEventCreateWithFlags(&h->ev_rerun[b], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_rev[b], hipEventDisableTiming) != hipSuccess) {
+      athmi_adjoint_destroy(h);
+      return fail(ATHMI_EHIP, "athmi_adjoint_create: event creation failed");
+    }
+  if (hipEventCreateWithFlags(&h->ev_sync, hipEventDisableTiming) != hipSuccess) {
+    athmi_adjoint_destroy(h);
+    return fail(ATHMI_EHIP, "athmi_adjoint_create: event creation failed");
+  }
   h->ck = h->mem;
   h->seg = h->ck + h->state_doubles * h->nck;
-  h->adj[0] = h->seg + h->state_doubles * (h->every + 1);
+  h->adj[0] = h->seg + h->state_doubles * h->nbank * (h->every + 1);
   h->adj[1] = h->adj[0] + h->state_doubles;
   h->ab = h->adj[1] + h->state_doubles;
   h->segsc = h->ab + (size_t)NADJ * I.ps;
-  std::vector<Inst> slots(2 * h->every, I);
-  for (int p = 0; p < 2; p++)
-    for (int m = 0; m < h->every; m++) {
-      Inst& S = slots[p * h->every + m];
-      const int cur = p ^ (m & 1);
-      S.state[cur] = h->seg + (size_t)m * h->state_doubles;
-      S.state[cur ^ 1] = h->seg + (size_t)(m + 1) * h->state_doubles;
-      S.scratch = h->segsc + (size_t)m * h->slot_doubles;
-      S.prims = S.scratch + (size_t)kSlotPrims * I.ps;
-    }
+  std::vector<Inst> slots(4 * h->every, I);
+  for (int b = 0; b < h->nbank; b++)
+    for (int p = 0; p < 2; p++)
+      for (int m = 0; m < h->every; m++) {
+        Inst& S = slots[(2 * b + p) * h->every + m];
+        const int cur = p ^ (m & 1);
+        S.state[cur] = h->seg_state(b, m);
+        S.state[cur ^ 1] = h->seg_state(b, m + 1);
+        S.scratch = h->slot(b, m);
+        S.prims = S.scratch + (size_t)kSlotPrims * I.ps;
+      }
   if (hipMemcpy(h->dslot, slots.data(), sizeof(Inst) * slots.size(), hipMemcpyHostToDevice) !=
       hipSuccess) {
     athmi_adjoint_destroy(h);
@@ -171,8 +223,15 @@ int athmi_adjoint_create(const athmi_desc* desc, double dt, int nsteps, int ever
 int athmi_adjoint_destroy(athmi_adjoint* h) {
   if (!h) return ATHMI_OK;
   if (h->g && h->g->stream) (void)hipStreamSynchronize(h->g->stream);
+  if (h->astream) (void)hipStreamSynchronize(h->astream);
   if (h->mem) (void)hipFree(h->mem);
   if (h->dslot) (void)hipFree(h->dslot);
+  for (int b = 0; b < 2; b++) {
+    if (h->ev_rerun[b]) (void)hipEventDestroy(h->ev_rerun[b]);
+    if (h->ev_rev[b]) (void)hipEventDestroy(h->ev_rev[b]);
+  }
+  if (h->ev_sync) (void)hipEventDestroy(h->ev_sync);
+  if (h->astream) (void)hipStreamDestroy(h->astream);
   if (h->g) athmi_destroy(h->g);
   delete h;
   return ATHMI_OK;
@@ -186,7 +245,7 @@ int athmi_adjoint_set_stream(athmi_adjoint* h, void* hip_stream) {
 int athmi_adjoint_forward(athmi_adjoint* h, const double* state0, double* stateN) {
   if (!h || !state0 || !stateN) return fail(ATHMI_EINVAL, "athmi_adjoint_forward: null argument");
   int rc;
-  if ((rc = dense_to_planes(h, live_state(h), state0))) return rc;
+  if ((rc = dense_to_planes(h, live_state(h), state0, h->g->stream))) return rc;
   HIP_TRY(h->g->launch(&h->g->bl, SK_GHOST_CUR, h->g->stream));
   for (int n = 0; n < h->nsteps; n++) {
     if (n % h->every == 0 &&
@@ -194,7 +253,7 @@ int athmi_adjoint_forward(athmi_adjoint* h, const double* state0, double* stateN
       return rc;
     if ((rc = rollout_step(h))) return rc;
   }
-  if ((rc = planes_to_dense(h, stateN, live_state(h)))) return rc;
+  if ((rc = planes_to_dense(h, stateN, live_state(h), h->g->stream))) return rc;
   HIP_TRY(hipStreamSynchronize(h->g->stream));
   h->have_forward = true;
   return ATHMI_OK;
@@ -205,52 +264,54 @@ int athmi_adjoint_backward(athmi_adjoint* h, const double* gradN, double* grad0)
   if (!h->have_forward)
     return fail(ATHMI_ESTATE, "athmi_adjoint_backward: run athmi_adjoint_forward first");
   int rc;
+  hipStream_t R = h->g->stream, A = h->astream;
   double* aout = h->adj[0];
   double* ain = h->adj[1];
-  if ((rc = dense_to_planes(h, aout, gradN))) return rc;
+  if ((rc = dense_to_planes(h, aout, gradN, R))) return rc;
   AdjArgs a;
   a.g = h->g->hinst[0];
   a.ab = h->ab;
   a.dt = h->dt;
   a.ain = ain;
-  HIP_TRY(athmi_adjoint_launch(&a, AK_CLEAR, h->g->stream));
+  HIP_TRY(athmi_adjoint_launch(&a, AK_CLEAR, R));
+  HIP_TRY(hipEventRecord(h->ev_sync, R));
+  HIP_TRY(hipStreamWaitEvent(A, h->ev_sync, 0));
   const size_t ps = a.g.ps;
+  bool rev_pending[2] = {false, false};
+  if ((rc = rerun_segment(h, h->nck - 1, (h->nck - 1) % h->nbank))) return rc;
   for (int s = h->nck - 1; s >= 0; s--) {
+    const int b = s % h->nbank;
     const int n0 = s * h->every, n1 = std::min(n0 + h->every, h->nsteps);
-    // re-run the segment from its checkpoint into the per-step slots
-    const Inst* slot = h->dslot + (size_t)h->cur * h->every;
-    if ((rc = copy_state(h, h->seg, h->ck + (size_t)s * h->state_doubles))) return rc;
-    for (int m = n0; m < n1; m++) {
-      const Inst* ins = slot + (m - n0);
-      double* sc = h->segsc + (size_t)(m - n0) * h->slot_doubles;
-      if ((rc = issue_on(h, ins, kSeqForwardScratch, sizeof(kSeqForwardScratch) / sizeof(int))))
-        return rc;
-      HIP_TRY(hipMemcpyAsync(sc + kSlotEmf * ps, sc + S_EMF * ps, ps * sizeof(double),
-                             hipMemcpyDeviceToDevice, h->g->stream));
-      if (m + 1 < n1) {
-        if ((rc = issue_on(h, ins, kSeqFinishStep, sizeof(kSeqFinishStep) / sizeof(int))))
-          return rc;
-        h->cur ^= 1;  // k_end swaps the buffers: the output is seg[m + 1]
-        if ((rc = issue_on(h, ins, kSeqGhostCur, 1))) return rc;
-      }
-    }
+    // reverse segment s on the reversal stream once its re-run is done
+    HIP_TRY(hipEventRecord(h->ev_rerun[b], R));
+    HIP_TRY(hipStreamWaitEvent(A, h->ev_rerun[b], 0));
     for (int m = n1 - 1; m >= n0; m--) {
-      const double* sc = h->segsc + (size_t)(m - n0) * h->slot_doubles;
-      a.st = h->seg + (size_t)(m - n0) * h->state_doubles;
+      const double* sc = h->slot(b, m - n0);
+      a.st = h->seg_state(b, m - n0);
       a.sc = sc;
       a.emfp = sc + kSlotEmf * ps;
       a.pw = sc + kSlotPrims * ps;
       a.ain = ain;
       a.aout = aout;
-      for (int k : kSeqReverse) HIP_TRY(athmi_adjoint_launch(&a, k, h->g->stream));
+      for (int k : kSeqReverse) HIP_TRY(athmi_adjoint_launch(&a, k, A));
       std::swap(ain, aout);
+    }
+    HIP_TRY(hipEventRecord(h->ev_rev[b], A));
+    rev_pending[b] = true;
+    // meanwhile re-run segment s-1 into its bank, once that bank's last
+    // reversal is done
+    if (s > 0) {
+      const int bn = (s - 1) % h->nbank;
+      if (rev_pending[bn]) HIP_TRY(hipStreamWaitEvent(R, h->ev_rev[bn], 0));
+      if ((rc = rerun_segment(h, s - 1, bn))) return rc;
     }
   }
   // the set_bvals that precedes the first step
   a.aout = aout;
-  HIP_TRY(athmi_adjoint_launch(&a, AK_FOLD_OUT, h->g->stream));
-  if ((rc = planes_to_dense(h, grad0, aout))) return rc;
-  HIP_TRY(hipStreamSynchronize(h->g->stream));
+  HIP_TRY(athmi_adjoint_launch(&a, AK_FOLD_OUT, A));
+  if ((rc = planes_to_dense(h, grad0, aout, A))) return rc;
+  HIP_TRY(hipStreamSynchronize(A));
+  HIP_TRY(hipStreamSynchronize(R));
   return ATHMI_OK;
 }
 
