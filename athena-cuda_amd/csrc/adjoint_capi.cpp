@@ -28,9 +28,13 @@ const int kSeqForwardScratch[] = {SK_BEGIN_ONCE, SK_PRED_X1, SK_PRED_X2, SK_CORN
 // the rest of integrate_2d after kSeqForwardScratch
 const int kSeqFinishStep[] = {SK_CORNER_C, SK_FINAL, SK_GHOST_CARRY, SK_END_ONCE};
 const int kSeqGhostCur[] = {SK_GHOST_CUR};
-const int kSeqReverse[] = {AK_FOLD_OUT,  AK_ZERO,       AK_UPDATE,   AK_CORNER_C, AK_RIEMANN_C,
-                           AK_HALF,      AK_TRANSVERSE, AK_CORNER_P, AK_EMF_CC,   AK_PRED_X1,
-                           AK_PLM_X1,    AK_PRED_X2,    AK_PLM_X2};
+// the reverse of one step on two streams: the x1 and x2 halves of Steps 8
+// and 1-2 are independent (their adjoints meet in ain through atomic adds)
+const int kRevHead[] = {AK_FOLD_OUT, AK_ZERO, AK_UPDATE, AK_CORNER_C};  // then fork
+const int kRevMainA[] = {AK_RIEMANN_C1, AK_HALF};                       // || AK_RIEMANN_C2
+const int kRevMid[] = {AK_TRANSVERSE, AK_CORNER_P, AK_EMF_CC};          // after the join
+const int kRevTailA[] = {AK_PRED_X1, AK_PLM_X1};                        // || x2 tail
+const int kRevTailB[] = {AK_PRED_X2, AK_PLM_X2};
 // a slot of the segment's forward scratch: the step's NSCRATCH planes, its
 // predictor corner emf (the corrector's corner kernel overwrites S_EMF) and
 // the predictors' NPRIMS primitive face states
@@ -59,8 +63,9 @@ struct athmi_adjoint {
   // first step reads state buffer p is dslot[(2 b + p) K + m] (state buffers
   // seg[b][m], seg[b][m+1], scratch slot [b][m])
   Inst* dslot = nullptr;
-  hipStream_t astream = nullptr;           // the reversal's stream
+  hipStream_t astream = nullptr, bstream = nullptr;  // the reversal's two streams
   hipEvent_t ev_rerun[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr}, ev_sync = nullptr;
+  hipEvent_t ev_fork1 = nullptr, ev_join1 = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
 
   double* seg_state(int b, int m) const {
     return seg + ((size_t)b * (every + 1) + m) * state_doubles;
@@ -184,7 +189,12 @@ int athmi_adjoint_create(const athmi_desc* desc, double dt, int nsteps, int ever
       athmi_adjoint_destroy(h);
       return fail(ATHMI_EHIP, "athmi_adjoint_create: event creation failed");
     }
-  if (hipEventCreateWithFlags(&h->ev_sync, hipEventDisableTiming) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&h->bstream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_sync, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_fork1, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_join1, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_fork2, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_join2, hipEventDisableTiming) != hipSuccess) {
     athmi_adjoint_destroy(h);
     return fail(ATHMI_EHIP, "athmi_adjoint_create: event creation failed");
   }
@@ -230,8 +240,11 @@ int athmi_adjoint_destroy(athmi_adjoint* h) {
     if (h->ev_rerun[b]) (void)hipEventDestroy(h->ev_rerun[b]);
     if (h->ev_rev[b]) (void)hipEventDestroy(h->ev_rev[b]);
   }
-  if (h->ev_sync) (void)hipEventDestroy(h->ev_sync);
+  for (hipEvent_t e : {h->ev_sync, h->ev_fork1, h->ev_join1, h->ev_fork2, h->ev_join2})
+    if (e) (void)hipEventDestroy(e);
+  if (h->bstream) (void)hipStreamSynchronize(h->bstream);
   if (h->astream) (void)hipStreamDestroy(h->astream);
+  if (h->bstream) (void)hipStreamDestroy(h->bstream);
   if (h->g) athmi_destroy(h->g);
   delete h;
   return ATHMI_OK;
@@ -264,7 +277,7 @@ int athmi_adjoint_backward(athmi_adjoint* h, const double* gradN, double* grad0)
   if (!h->have_forward)
     return fail(ATHMI_ESTATE, "athmi_adjoint_backward: run athmi_adjoint_forward first");
   int rc;
-  hipStream_t R = h->g->stream, A = h->astream;
+  hipStream_t R = h->g->stream, A = h->astream, B = h->bstream;
   double* aout = h->adj[0];
   double* ain = h->adj[1];
   if ((rc = dense_to_planes(h, aout, gradN, R))) return rc;
@@ -276,6 +289,7 @@ int athmi_adjoint_backward(athmi_adjoint* h, const double* gradN, double* grad0)
   HIP_TRY(athmi_adjoint_launch(&a, AK_CLEAR, R));
   HIP_TRY(hipEventRecord(h->ev_sync, R));
   HIP_TRY(hipStreamWaitEvent(A, h->ev_sync, 0));
+  HIP_TRY(hipEventRecord(h->ev_join2, A));  // nothing pending on B yet
   const size_t ps = a.g.ps;
   bool rev_pending[2] = {false, false};
   if ((rc = rerun_segment(h, h->nck - 1, (h->nck - 1) % h->nbank))) return rc;
@@ -293,9 +307,24 @@ int athmi_adjoint_backward(athmi_adjoint* h, const double* gradN, double* grad0)
       a.pw = sc + kSlotPrims * ps;
       a.ain = ain;
       a.aout = aout;
-      for (int k : kSeqReverse) HIP_TRY(athmi_adjoint_launch(&a, k, A));
+      // the previous step's x2 tail on B wrote ain planes this step folds
+      HIP_TRY(hipStreamWaitEvent(A, h->ev_join2, 0));
+      for (int k : kRevHead) HIP_TRY(athmi_adjoint_launch(&a, k, A));
+      HIP_TRY(hipEventRecord(h->ev_fork1, A));
+      HIP_TRY(hipStreamWaitEvent(B, h->ev_fork1, 0));
+      HIP_TRY(athmi_adjoint_launch(&a, AK_RIEMANN_C2, B));
+      HIP_TRY(hipEventRecord(h->ev_join1, B));
+      for (int k : kRevMainA) HIP_TRY(athmi_adjoint_launch(&a, k, A));
+      HIP_TRY(hipStreamWaitEvent(A, h->ev_join1, 0));
+      for (int k : kRevMid) HIP_TRY(athmi_adjoint_launch(&a, k, A));
+      HIP_TRY(hipEventRecord(h->ev_fork2, A));
+      HIP_TRY(hipStreamWaitEvent(B, h->ev_fork2, 0));
+      for (int k : kRevTailB) HIP_TRY(athmi_adjoint_launch(&a, k, B));
+      HIP_TRY(hipEventRecord(h->ev_join2, B));
+      for (int k : kRevTailA) HIP_TRY(athmi_adjoint_launch(&a, k, A));
       std::swap(ain, aout);
     }
+    HIP_TRY(hipStreamWaitEvent(A, h->ev_join2, 0));
     HIP_TRY(hipEventRecord(h->ev_rev[b], A));
     rev_pending[b] = true;
     // meanwhile re-run segment s-1 into its bank, once that bank's last

@@ -9,7 +9,7 @@
 //   fold        set_bvals: each ghost's adjoint is added to the cell it copies
 //   update      Steps 9, 11a, 11b, 13 -> corrector fluxes, corner emf, identity
 //   corner<C>   Step 9 corner emf -> corrector fluxes' By/Bz, emf3 at t^{n+1/2}
-//   riemann_c   Steps 8b/8c: corrector Riemann fluxes -> corrected face states
+//   riemann_c   Steps 8b / 8c: corrector Riemann fluxes -> corrected face states
 //   half        Step 7: emf3 at t^{n+1/2} -> half-step face fields, predictor fluxes
 //   transverse  Steps 5, 6 and the half-step face fields of Step 4
 //   corner<P>   Step 4 corner emf -> predictor fluxes' By/Bz, emf3 at t^n
@@ -292,16 +292,22 @@ __device__ __forceinline__ CorrCtx corr_ctx(const AdjArgs& a) {
   return CorrCtx{a.g, a.st, a.sc, a.emfp, a.g.ps, a.g.pitch, 0.5 * dtodx1, 0.5 * dtodx2};
 }
 
-// Steps 8b, 8c (:438-464) over [js-1,je+1] x [is-1,ie+1]
+// Step 8b (X1: x1 faces [js-1,je+1] x [is,ie+1], :438-447) or 8c (x2 faces
+// [js,je+1] x [is-1,ie+1], :455-464); the two are independent
+template <bool X1>
 __global__ void __launch_bounds__(kTileX* kTileY) ak_riemann_c(AdjArgs a) {
   const Inst& g = a.g;
   int j, i;
-  if (!range_ij(g.js - 1, g.je + 1, g.is - 1, g.ie + 1, j, i)) return;
+  if (X1) {
+    if (!range_ij(g.js - 1, g.je + 1, g.is, g.ie + 1, j, i)) return;
+  } else {
+    if (!range_ij(g.js, g.je + 1, g.is - 1, g.ie + 1, j, i)) return;
+  }
   const CorrCtx c = corr_ctx(a);
   const size_t ps = g.ps;
   const uint32_t k = (uint32_t)(j * g.pitch + i);
   const EosC eos{g.gamma, g.g1, g.g2};
-  if (i >= g.is) {
+  if (X1) {
     Cons7 L, R;
     c.states_x1(k, L, R);
     const double bx = c.b1h(k);
@@ -315,8 +321,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_riemann_c(AdjArgs a) {
     st7(a.ab + A_UL1 * ps, ps, k, Lb);
     st7(a.ab + A_UR1 * ps, ps, k, Rb);
     a.ab[A_B1H * ps + k] = bxb;
-  }
-  if (j >= g.js) {
+  } else {
     Cons7 L, R;
     c.states_x2(k, L, R);
     const double bx = c.b2h(k);
@@ -636,8 +641,11 @@ extern "C" hipError_t athmi_adjoint_launch(const AdjArgs* ap, int kernel, hipStr
     case AK_CORNER_C:
       hipLaunchKernelGGL(ak_corner<false>, grid2(g.js, g.je + 1, g.is, g.ie + 1), b2, 0, s, a);
       break;
-    case AK_RIEMANN_C:
-      hipLaunchKernelGGL(ak_riemann_c, grid2(g.js - 1, g.je + 1, g.is - 1, g.ie + 1), b2, 0, s, a);
+    case AK_RIEMANN_C1:
+      hipLaunchKernelGGL(ak_riemann_c<true>, grid2(g.js - 1, g.je + 1, g.is, g.ie + 1), b2, 0, s, a);
+      break;
+    case AK_RIEMANN_C2:
+      hipLaunchKernelGGL(ak_riemann_c<false>, grid2(g.js, g.je + 1, g.is - 1, g.ie + 1), b2, 0, s, a);
       break;
     case AK_HALF:
       hipLaunchKernelGGL(ak_half, grid2(g.js - 1, g.je + 1, g.is - 1, g.ie + 1), b2, 0, s, a);

@@ -60,7 +60,8 @@ enum AdjKernel {
   AK_CLEAR,         // clear all adjoint planes (start of a reverse pass)
   AK_UPDATE,        // Steps 9, 11a, 11b, 13
   AK_CORNER_C,      // Step 9 corner emf
-  AK_RIEMANN_C,     // Steps 8b, 8c (corrector fluxes)
+  AK_RIEMANN_C1,    // Step 8b (corrector x1 fluxes)
+  AK_RIEMANN_C2,    // Step 8c (corrector x2 fluxes)
   AK_HALF,          // Step 7 (d, M1, M2 and emf3 at t^{n+1/2})
   AK_TRANSVERSE,    // Steps 5, 6 and the half-step face fields of Step 4
   AK_CORNER_P,      // Step 4 corner emf
