@@ -33,8 +33,8 @@ const int kSeqGhostCur[] = {SK_GHOST_CUR};
 const int kRevHead[] = {AK_FOLD_OUT, AK_ZERO, AK_UPDATE, AK_CORNER_C};  // then fork
 const int kRevMainA[] = {AK_RIEMANN_C1, AK_HALF};                       // || AK_RIEMANN_C2
 const int kRevMid[] = {AK_TRANSVERSE, AK_CORNER_P, AK_EMF_CC};          // after the join
-const int kRevTailA[] = {AK_PRED_X1, AK_PLM_X1};                        // || x2 tail
-const int kRevTailB[] = {AK_PRED_X2, AK_PLM_X2};
+const int kRevTailA[] = {AK_PRED_X1, AK_PLM_X1, AK_PLMG_X1};            // || x2 tail
+const int kRevTailB[] = {AK_PRED_X2, AK_PLM_X2, AK_PLMG_X2};
 // a slot of the segment's forward scratch: the step's NSCRATCH planes, its
 // predictor corner emf (the corrector's corner kernel overwrites S_EMF) and
 // the predictors' NPRIMS primitive face states

@@ -15,7 +15,8 @@
 //   corner<P>   Step 4 corner emf -> predictor fluxes' By/Bz, emf3 at t^n
 //   emf_cc      Step 3
 //   pred<X1/X2> Steps 1c-1e / 2c-2e: sources, prim->cons, predictor fluxes
-//   plm_x1/x2   Steps 1a-1b / 2a-2b: cons->prim and the PLM reconstruction
+//   plm_x1/x2   Steps 1b / 2b: the PLM reconstruction
+//   plm_gather  Steps 1a / 2a: cons->prim
 // Nonlinear stages recompute their forward values with the reference's
 // evaluation order (this file is built with -ffp-contract=off, like the exact
 // forward kernels), so every branch is the one the forward pass took. Linear
@@ -560,8 +561,11 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_pred(AdjArgs a) {
   st7(a.ab + (X1 ? A_WR1 : A_WR2) * ps, ps, k, wrb);
 }
 
-// Steps 1a-1b / 2a-2b reversed for one cell of [js-2,je+2] x [is-2,ie+2]:
-// the PLM of cell k produced Wl at the next face and Wr at its own face.
+// Steps 1b / 2b reversed for one cell of [js-2,je+2] x [is-2,ie+2]: the PLM
+// of cell k produced Wl at the next face and Wr at its own face. Stores the
+// adjoints of the primitive states of cells k-1, k, k+1 and of Bxc[k] (22
+// planes; zeros where the outputs' adjoints are zero); plm_gather hands them
+// through cons->prim.
 template <bool X1>
 __global__ void __launch_bounds__(kTileX* kTileY) ak_plm(AdjArgs a) {
   const Inst& g = a.g;
@@ -570,6 +574,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_plm(AdjArgs a) {
   const size_t ps = g.ps;
   const uint32_t stride = X1 ? 1u : (uint32_t)g.pitch;
   const uint32_t k = (uint32_t)(j * g.pitch + i);
+  double* dst = a.ab + (X1 ? A_PW1 : A_PW2) * ps;
   double olb[7], orb[7];
   bool any = false;
   {
@@ -581,7 +586,10 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_plm(AdjArgs a) {
       any = any || olb[n] != 0.0 || orb[n] != 0.0;
     }
   }
-  if (!any) return;
+  if (!any) {
+    for (int n = 0; n < 22; n++) dst[n * ps + k] = 0.0;
+    return;
+  }
   const EosC eos{g.gamma, g.g1, g.g2};
   const double* st = a.st;
   // cell state in the sweep's order and its normal field
@@ -602,14 +610,48 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_plm(AdjArgs a) {
     for (int n = 0; n < 7; n++) wb[c][n] = 0.0;
   double bxcb = 0.0;
   P.back(olb, orb, wb[0], wb[1], wb[2], bxcb);
-  for (int c = 0; c < 3; c++) {
-    Cons7 ub = zero_cons();
-    double bxb = (c == 1) ? bxcb : 0.0;
-    const Prim7 Wb{wb[c][0], wb[c][1], wb[c][2], wb[c][3], wb[c][4], wb[c][5], wb[c][6]};
-    cons_to_prim_b(cons_at(kc[c]), st[qx * ps + kc[c]], eos, Wb, ub, bxb);
-    for (int n = 0; n < 7; n++) ADD(a.ain + q[n] * ps, kc[c], arr(ub)[n]);
-    ADD(a.ain + qx * ps, kc[c], bxb);
+  for (int c = 0; c < 3; c++)
+    for (int n = 0; n < 7; n++) dst[(c * 7 + n) * ps + k] = wb[c][n];
+  dst[21 * ps + k] = bxcb;
+}
+
+// Steps 1a / 2a reversed per cell: the primitive-state adjoint of cell c sums
+// the PLM reverses of cells c+1 (its left neighbour slot), c and c-1, and goes
+// through cons->prim once. X1: cells [js-2,je+2] x [is-3,ie+3]; x2:
+// [js-3,je+3] x [is-2,ie+2]. Atomic adds: the two sweeps run concurrently.
+template <bool X1>
+__global__ void __launch_bounds__(kTileX* kTileY) ak_plm_gather(AdjArgs a) {
+  const Inst& g = a.g;
+  int j, i;
+  if (X1) {
+    if (!range_ij(g.js - 2, g.je + 2, g.is - 3, g.ie + 3, j, i)) return;
+  } else {
+    if (!range_ij(g.js - 3, g.je + 3, g.is - 2, g.ie + 2, j, i)) return;
   }
+  const size_t ps = g.ps;
+  const uint32_t stride = X1 ? 1u : (uint32_t)g.pitch;
+  const uint32_t k = (uint32_t)(j * g.pitch + i);
+  const double* src = a.ab + (X1 ? A_PW1 : A_PW2) * ps;
+  double wb[7];
+  bool any = false;
+  for (int n = 0; n < 7; n++) {
+    wb[n] = src[n * ps + k + stride] + src[(7 + n) * ps + k] + src[(14 + n) * ps + k - stride];
+    any = any || wb[n] != 0.0;
+  }
+  double bxb = src[21 * ps + k];
+  if (!any && bxb == 0.0) return;
+  const EosC eos{g.gamma, g.g1, g.g2};
+  const double* st = a.st;
+  const int q[7] = {F_D, X1 ? F_M1 : F_M2, X1 ? F_M2 : F_M3, X1 ? F_M3 : F_M1, F_E,
+                    X1 ? F_B2C : F_B3C, X1 ? F_B3C : F_B1C};
+  const int qx = X1 ? F_B1C : F_B2C;
+  const Cons7 u{st[q[0] * ps + k], st[q[1] * ps + k], st[q[2] * ps + k], st[q[3] * ps + k],
+                st[q[4] * ps + k], st[q[5] * ps + k], st[q[6] * ps + k]};
+  Cons7 ub = zero_cons();
+  const Prim7 Wb{wb[0], wb[1], wb[2], wb[3], wb[4], wb[5], wb[6]};
+  cons_to_prim_b(u, st[qx * ps + k], eos, Wb, ub, bxb);
+  for (int n = 0; n < 7; n++) ADD(a.ain + q[n] * ps, k, arr(ub)[n]);
+  ADD(a.ain + qx * ps, k, bxb);
 }
 
 }  // namespace adjk
@@ -670,6 +712,14 @@ extern "C" hipError_t athmi_adjoint_launch(const AdjArgs* ap, int kernel, hipStr
       break;
     case AK_PLM_X2:
       hipLaunchKernelGGL(ak_plm<false>, grid2(g.js - 2, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);
+      break;
+    case AK_PLMG_X1:
+      hipLaunchKernelGGL(ak_plm_gather<true>, grid2(g.js - 2, g.je + 2, g.is - 3, g.ie + 3), b2, 0,
+                         s, a);
+      break;
+    case AK_PLMG_X2:
+      hipLaunchKernelGGL(ak_plm_gather<false>, grid2(g.js - 3, g.je + 3, g.is - 2, g.ie + 2), b2, 0,
+                         s, a);
       break;
     default:
       return hipErrorInvalidValue;

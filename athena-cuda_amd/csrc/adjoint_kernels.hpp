@@ -37,7 +37,10 @@ enum AdjPlane {
   A_HM2 = 92,
   A_HB1 = 93,   // ... and the cell-centred half-step B1c, B2c
   A_HB2 = 94,
-  NADJ = 95
+  A_PW1 = 95,   // PLM reverse per cell (x1 sweep): adjoints of its 3 stencil cells'
+                // primitive states (3 x 7) and of its Bxc (1)
+  A_PW2 = 117,  // ... x2 sweep
+  NADJ = 139
 };
 
 // Everything the adjoint kernels of one step need
@@ -67,9 +70,11 @@ enum AdjKernel {
   AK_CORNER_P,      // Step 4 corner emf
   AK_EMF_CC,        // Step 3
   AK_PRED_X1,       // Steps 1c-1e (source, prim->cons, predictor flux) per x1 face
-  AK_PLM_X1,        // Steps 1a-1b (cons->prim, PLM) per cell
+  AK_PLM_X1,        // Step 1b (PLM) per cell
+  AK_PLMG_X1,       // Step 1a (cons->prim) per cell, gathering the PLM adjoints
   AK_PRED_X2,       // Steps 2c-2e per x2 face
-  AK_PLM_X2,        // Steps 2a-2b per cell
+  AK_PLM_X2,        // Step 2b per cell
+  AK_PLMG_X2,       // Step 2a per cell
   AK_COUNT
 };
 
