@@ -23,7 +23,7 @@
 namespace {
 
 // forward scratch of one step: predictor, predictor corner emf, corrector fluxes
-const int kSeqForwardScratch[] = {SK_BEGIN_ONCE, SK_PRED_X1, SK_PRED_X2, SK_CORNER_P, SK_CORR,
+const int kSeqForwardScratch[] = {SK_BEGIN_ONCE, SK_PRED_X1_SAVE, SK_PRED_X2_SAVE, SK_CORNER_P, SK_CORR,
                                   SK_CORR2};
 // the rest of integrate_2d after kSeqForwardScratch
 const int kSeqFinishStep[] = {SK_CORNER_C, SK_FINAL, SK_GHOST_CARRY, SK_END_ONCE};

@@ -121,6 +121,7 @@ __device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double 
 // thread converts one cell to primitives; neighbours are exchanged in LDS for
 // the characteristic PLM (lr_states_plm.c:95-245); each face then gets the
 // 1/2 dt MHD source on By, prim->cons and the Riemann flux.
+template <bool SAVE>  // SAVE: also store the primitive face states (Inst::prims)
 __global__ void __launch_bounds__(kX1Block) k_pred_x1(const Inst* __restrict__ insts, int ninst,
                                                       int ntiles, int remap) {
   __shared__ double sw[7][kX1Block];
@@ -188,7 +189,7 @@ __global__ void __launch_bounds__(kX1Block) k_pred_x1(const Inst* __restrict__ i
       Prim7 wl, wr;
       double b0;
       build(ul, ur, wl, wr, b0);
-      if (g.prims != nullptr) {
+      if (SAVE) {
         put7(g.prims + P_WL1 * ps, ps, k, wl);
         put7(g.prims + P_WR1 * ps, ps, k, wr);
       }
@@ -205,6 +206,7 @@ __global__ void __launch_bounds__(kX1Block) k_pred_x1(const Inst* __restrict__ i
 // order (d, M2, M3, M1, E, B3c, B1c) with Bx = B2c. Lanes own adjacent
 // columns (coalesced rows); each lane marches down a segment of faces keeping
 // the three-cell PLM stencil and the pending left state in registers.
+template <bool SAVE>
 __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ insts, int ninst,
                                                       int ntiles, int seg_len, int remap) {
   __shared__ double sxl[7][kX2Block];
@@ -272,7 +274,7 @@ __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ i
       Prim7 wl, wr;
       double b0;
       build(ul, ur, wl, wr, b0);
-      if (g.prims != nullptr) {
+      if (SAVE) {
         put7(g.prims + P_WL2 * ps, ps, k, wl);
         put7(g.prims + P_WR2 * ps, ps, k, wr);
       }
@@ -773,11 +775,20 @@ hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
                          kernel == SK_BEGIN_STEP ? 0 : 1);
       break;
     case SK_PRED_X1:
-      hipLaunchKernelGGL(k_pred_x1, dim3(grid_of(b, K_PRED_X1, 4)), dim3(kX1Block), 0, s,
+      hipLaunchKernelGGL(k_pred_x1<false>, dim3(grid_of(b, K_PRED_X1, 4)), dim3(kX1Block), 0, s,
+                         b.insts, b.ninst, b.ntiles[K_PRED_X1], b.shape.xcd_remap);
+      break;
+    case SK_PRED_X1_SAVE:
+      hipLaunchKernelGGL(k_pred_x1<true>, dim3(grid_of(b, K_PRED_X1, 4)), dim3(kX1Block), 0, s,
                          b.insts, b.ninst, b.ntiles[K_PRED_X1], b.shape.xcd_remap);
       break;
     case SK_PRED_X2:
-      hipLaunchKernelGGL(k_pred_x2, dim3(grid_of(b, K_PRED_X2, 16)), dim3(kX2Block), 0, s,
+      hipLaunchKernelGGL(k_pred_x2<false>, dim3(grid_of(b, K_PRED_X2, 16)), dim3(kX2Block), 0, s,
+                         b.insts, b.ninst, b.ntiles[K_PRED_X2], b.shape.x2_seg,
+                         b.shape.xcd_remap);
+      break;
+    case SK_PRED_X2_SAVE:
+      hipLaunchKernelGGL(k_pred_x2<true>, dim3(grid_of(b, K_PRED_X2, 16)), dim3(kX2Block), 0, s,
                          b.insts, b.ninst, b.ntiles[K_PRED_X2], b.shape.x2_seg,
                          b.shape.xcd_remap);
       break;

@@ -51,7 +51,7 @@ struct Inst {
   double dx1, dx2, gamma, g1, g2;
   double* state[2];   // two ping-pong buffers of NSTATE planes
   double* scratch;    // NSCRATCH planes
-  double* prims;      // NPRIMS planes written by the predictors when non-null
+  double* prims;      // NPRIMS planes written by SK_PRED_X1_SAVE / SK_PRED_X2_SAVE
   size_t ps;          // plane stride in doubles
   Clock* clk;
   // first global tile of this instance for each kernel kind
@@ -134,6 +134,8 @@ enum StepKernel {
   SK_DTI,             // new_dt partial maxima
   SK_DT_FINAL,        // new_dt.c:93-97
   SK_DIVB,
+  SK_PRED_X1_SAVE,    // predictors that also store their primitive face states
+  SK_PRED_X2_SAVE,    // (Inst::prims), for the adjoint's re-run
   SK_COUNT
 };
 
