@@ -245,11 +245,25 @@ __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ i
     Prim7 wc = prim_at(jf0, bxc);
     Prim7 wl_cur, wr_unused;
     plm_cell(wa, wb, wc, bxb, dtodx2, eos, wl_cur, wr_unused);
+    // the conserved state of the row after next is loaded one iteration
+    // ahead, so its latency hides behind the current face's work (the last
+    // iteration reads at most row je+4, the outermost ghost row)
+    auto raw_at = [&](int jj, Cons7& u, double& bx) {
+      const uint32_t k = (uint32_t)(jj * pitch + i);
+      u = Cons7{LD(st + F_D * ps, k), LD(st + F_M2 * ps, k), LD(st + F_M3 * ps, k), LD(st + F_M1 * ps, k),
+                LD(st + F_E * ps, k), LD(st + F_B3C * ps, k), LD(st + F_B1C * ps, k)};
+      bx = LD(st + F_B2C * ps, k);
+    };
+    Cons7 un;
+    double bxn;
+    raw_at(jf0 + 1, un, bxn);
     for (int j = jf0; j <= jf1; j++) {
       wa = wb;
       wb = wc;
       bxb = bxc;
-      wc = prim_at(j + 1, bxc);
+      wc = cons_to_prim(un, bxn, eos);
+      bxc = bxn;
+      raw_at(j + 2, un, bxn);
       Prim7 wl_next, wr_cur;
       plm_cell(wa, wb, wc, bxb, dtodx2, eos, wl_next, wr_cur);
       const uint32_t k = (uint32_t)(j * pitch + i);
