@@ -68,6 +68,8 @@ SIGNATURES = {
     "athmi_use_graph": (_I, [_P, _I]),
     "athmi_autotune": (_I, [_P, _I, _IP, _IP, _IP, _DP]),
     "athmi_set_tile_order": (_I, [_P, _I]),
+    "athmi_set_corr_rows": (_I, [_P, _I]),
+    "athmi_get_shape": (_I, [_P, _IP, _IP, _IP, _IP]),
     "athmi_device_state": (_I, [_P, _I, ctypes.POINTER(_P), _IP]),
     "athmi_pgen_blast": (_I, [_P, _I, _D, _D, _D, _D, _D, _D, _D]),
     "athmi_pgen_field_loop": (_I, [_P, _I, _D, _D, _D, _I]),
@@ -265,6 +267,15 @@ class Grid:
         _check(lib().athmi_autotune(self._h, steps, ctypes.byref(a), ctypes.byref(b),
                                     ctypes.byref(c), ctypes.byref(ms)), "autotune")
         return a.value, b.value, c.value, ms.value
+
+    def set_corr_rows(self, rows=4):
+        _check(lib().athmi_set_corr_rows(self._h, int(rows)), "set_corr_rows")
+
+    def launch_shape(self):
+        """(x2_seg, persist, xcd_remap, corr_rows) of the current launch shape"""
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(lib().athmi_get_shape(self._h, *[ctypes.byref(x) for x in v]), "get_shape")
+        return tuple(x.value for x in v)
 
     def set_tile_order(self, xcd_remap=1):
         _check(lib().athmi_set_tile_order(self._h, int(xcd_remap)), "set_tile_order")

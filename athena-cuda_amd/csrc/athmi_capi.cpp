@@ -64,6 +64,7 @@ int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
   g->bl.shape.x2_seg = (int)std::max(4L, std::min(16L, cols / 2048));
   g->bl.shape.persist = 0;
   g->bl.shape.xcd_remap = 1;
+  g->bl.shape.corr_rows = kTileY;
   // device memory: per instance 2*NSTATE state planes + NSCRATCH scratch planes
   std::vector<size_t> off(n);
   size_t total = 0;
@@ -403,10 +404,11 @@ int athmi_set_shape(athmi_grid* g, int x2_seg, int persist) {
   return g->refresh_tiles();
 }
 
-// Launch-shape autotuner: every candidate (x2 marching depth x persistent
-// occupancy) runs the same `steps` main-loop steps from the same state, timed
-// with HIP events on the handle's stream through the step graph; the state
-// and clocks are restored after each candidate and the fastest shape is kept.
+// Launch-shape autotuner: every candidate (tile order x x2 marching depth x
+// persistent occupancy, then the corrector's tile height) runs the same
+// `steps` main-loop steps from the same state, timed with HIP events on the
+// handle's stream through the step graph; the state and clocks are restored
+// after each candidate and the fastest shape is kept.
 int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd_remap,
                    double* ms_per_step) {
   if (!g || steps <= 0) return fail(ATHMI_EINVAL, "athmi_autotune: bad arguments");
@@ -444,30 +446,43 @@ int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd
   LaunchShape best_shape = old;
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = fail(ATHMI_EHIP, "athmi_autotune: event creation failed");
-  for (int c = 0; !rc && c < 2; c++)
-  for (int a = 0; !rc && a < (int)(sizeof(kSeg) / sizeof(int)); a++) {
-    for (int b = 0; !rc && b < (int)(sizeof(kPersist) / sizeof(int)); b++) {
-      g->bl.shape.x2_seg = kSeg[a];
-      g->bl.shape.persist = kPersist[b];
-      g->bl.shape.xcd_remap = c;
-      if ((rc = g->refresh_tiles())) break;
-      if ((rc = athmi_step(g))) break;  // capture + first replay, untimed
-      if ((rc = copy_state(false))) break;
-      float ms = 0.f;
-      if (hipEventRecord(e0, g->stream) != hipSuccess) { rc = fail(ATHMI_EHIP, "event"); break; }
-      for (int s = 0; !rc && s < steps; s++) rc = athmi_step(g);
-      if (rc) break;
-      if (hipEventRecord(e1, g->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        rc = fail(ATHMI_EHIP, "athmi_autotune: event timing failed");
-        break;
-      }
-      if (ms / steps < best) {
-        best = ms / steps;
-        best_shape = g->bl.shape;
-      }
-      rc = copy_state(false);
+  // time one shape from the saved state; restores state and clocks
+  auto trial = [&](const LaunchShape& sh) -> int {
+    g->bl.shape = sh;
+    int r;
+    if ((r = g->refresh_tiles())) return r;
+    if ((r = athmi_step(g))) return r;  // capture + first replay, untimed
+    if ((r = copy_state(false))) return r;
+    float ms = 0.f;
+    if (hipEventRecord(e0, g->stream) != hipSuccess) return fail(ATHMI_EHIP, "event");
+    for (int s = 0; s < steps; s++)
+      if ((r = athmi_step(g))) return r;
+    if (hipEventRecord(e1, g->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      return fail(ATHMI_EHIP, "athmi_autotune: event timing failed");
+    if (ms / steps < best) {
+      best = ms / steps;
+      best_shape = sh;
     }
+    return copy_state(false);
+  };
+  // tile order x x2 marching depth x persistent occupancy, then the
+  // corrector's tile height at the best of those
+  for (int c = 0; !rc && c < 2; c++)
+    for (int a = 0; !rc && a < (int)(sizeof(kSeg) / sizeof(int)); a++)
+      for (int b = 0; !rc && b < (int)(sizeof(kPersist) / sizeof(int)); b++) {
+        LaunchShape sh = old;
+        sh.x2_seg = kSeg[a];
+        sh.persist = kPersist[b];
+        sh.xcd_remap = c;
+        rc = trial(sh);
+      }
+  const LaunchShape base = best_shape;
+  for (int rows : {4, 2, 1}) {
+    if (rc || rows == base.corr_rows) continue;
+    LaunchShape sh = base;
+    sh.corr_rows = rows;
+    rc = trial(sh);
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
@@ -482,6 +497,25 @@ int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd
   if (persist) *persist = best_shape.persist;
   if (xcd_remap) *xcd_remap = best_shape.xcd_remap;
   if (ms_per_step) *ms_per_step = best;
+  return ATHMI_OK;
+}
+
+int athmi_set_corr_rows(athmi_grid* g, int rows) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  if (rows != 1 && rows != 2 && rows != 4)
+    return fail(ATHMI_EINVAL, "athmi_set_corr_rows: rows must be 1, 2 or 4");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->bl.shape.corr_rows = rows;
+  return g->refresh_tiles();
+}
+
+int athmi_get_shape(const athmi_grid* g, int* x2_seg, int* persist, int* xcd_remap,
+                    int* corr_rows) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  if (x2_seg) *x2_seg = g->bl.shape.x2_seg;
+  if (persist) *persist = g->bl.shape.persist;
+  if (xcd_remap) *xcd_remap = g->bl.shape.xcd_remap;
+  if (corr_rows) *corr_rows = g->bl.shape.corr_rows;
   return ATHMI_OK;
 }
 

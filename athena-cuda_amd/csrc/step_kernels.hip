@@ -362,9 +362,9 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_corner(const Inst* __restric
 // PART 1: the x1 faces (Steps 5, 8b); PART 2: the x2 faces and Step 7 (6, 8c, 7).
 // Split so that each thread evaluates one Riemann flux: the loads of the second
 // face no longer compete for registers with the first face's flux.
-template <int PART>
-__global__ void __launch_bounds__(kTileX* kTileY) k_corr(const Inst* __restrict__ insts,
-                                                             int ninst, int ntiles, int remap) {
+template <int PART, int TY>  // TY: rows of a tile of kTileX x TY faces
+__global__ void __launch_bounds__(kTileX* TY) k_corr(const Inst* __restrict__ insts,
+                                                         int ninst, int ntiles, int remap) {
   for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
     const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, K_CORR, t)];
@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_corr(const Inst* __restrict_
     const int ntx = dcdiv(g.nx1 + 3, kTileX);
     const int ty = local / ntx, tx = local - ty * ntx;
     const int i = g.is - 1 + tx * kTileX + threadIdx.x;
-    const int j = g.js - 1 + ty * kTileY + threadIdx.y;
+    const int j = g.js - 1 + ty * TY + threadIdx.y;
     if (i > g.ie + 1 || j > g.je + 1) continue;
     const double dt = clk->dt;
     const double dtodx1 = dt / g.dx1, dtodx2 = dt / g.dx2;
@@ -811,13 +811,23 @@ hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
                          b.ninst, b.ntiles[K_CORNER_P], b.shape.xcd_remap);
       break;
     case SK_CORR:
-      hipLaunchKernelGGL(k_corr<1>, dim3(grid_of(b, K_CORR, 4)), b2, 0, s, b.insts, b.ninst,
-                         b.ntiles[K_CORR], b.shape.xcd_remap);
+    case SK_CORR2: {
+      // the same threads in flight per CU whatever the tile height
+      const int ty = b.shape.corr_rows;
+      const dim3 grid(grid_of(b, K_CORR, 4 * (kTileY / ty))), blk(kTileX, ty);
+      const bool x1 = (kernel == SK_CORR);
+      if (ty == 1) {
+        if (x1) hipLaunchKernelGGL((k_corr<1, 1>), grid, blk, 0, s, b.insts, b.ninst, b.ntiles[K_CORR], b.shape.xcd_remap);
+        else hipLaunchKernelGGL((k_corr<2, 1>), grid, blk, 0, s, b.insts, b.ninst, b.ntiles[K_CORR], b.shape.xcd_remap);
+      } else if (ty == 2) {
+        if (x1) hipLaunchKernelGGL((k_corr<1, 2>), grid, blk, 0, s, b.insts, b.ninst, b.ntiles[K_CORR], b.shape.xcd_remap);
+        else hipLaunchKernelGGL((k_corr<2, 2>), grid, blk, 0, s, b.insts, b.ninst, b.ntiles[K_CORR], b.shape.xcd_remap);
+      } else {
+        if (x1) hipLaunchKernelGGL((k_corr<1, 4>), grid, blk, 0, s, b.insts, b.ninst, b.ntiles[K_CORR], b.shape.xcd_remap);
+        else hipLaunchKernelGGL((k_corr<2, 4>), grid, blk, 0, s, b.insts, b.ninst, b.ntiles[K_CORR], b.shape.xcd_remap);
+      }
       break;
-    case SK_CORR2:
-      hipLaunchKernelGGL(k_corr<2>, dim3(grid_of(b, K_CORR, 4)), b2, 0, s, b.insts, b.ninst,
-                         b.ntiles[K_CORR], b.shape.xcd_remap);
-      break;
+    }
     case SK_CORNER_C:
       hipLaunchKernelGGL(k_corner<false>, dim3(grid_of(b, K_CORNER_C, 8)), b2, 0, s, b.insts,
                          b.ninst, b.ntiles[K_CORNER_C], b.shape.xcd_remap);

@@ -83,12 +83,12 @@ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // number of tiles of kind k for a grid of nx1 x nx2 zones (x2seg: faces
 // marched per thread by the x2 predictor)
-inline int tiles_of(int k, int nx1, int nx2, int x2seg) {
+inline int tiles_of(int k, int nx1, int nx2, int x2seg, int corr_rows) {
   const int sx = nx1 + 8;
   switch (k) {
     case K_PRED_X1: return (nx2 + 4) * cdiv(nx1 + 4, kX1Faces);
     case K_PRED_X2: return cdiv(nx1 + 5, kX2Block) * cdiv(nx2 + 4, x2seg);
-    case K_CORR: return cdiv(nx1 + 3, kTileX) * cdiv(nx2 + 3, kTileY);
+    case K_CORR: return cdiv(nx1 + 3, kTileX) * cdiv(nx2 + 3, corr_rows);
     case K_CORNER_P: return cdiv(nx1 + 4, kTileX) * cdiv(nx2 + 4, kTileY);
     case K_CORNER_C: return cdiv(nx1 + 2, kTileX) * cdiv(nx2 + 2, kTileY);
     case K_FINAL: return cdiv(nx1 + 1, kTileX) * cdiv(nx2 + 1, kTileY);
@@ -102,6 +102,7 @@ struct LaunchShape {
   int x2_seg;        // faces marched per thread in the x2 predictor
   int persist;       // 0: one tile per workgroup; >0: workgroups per CU of a persistent grid
   int xcd_remap;     // 1: contiguous runs of tiles per XCD (L2 locality), 0: launch order
+  int corr_rows;     // rows of a corrector tile (kTileX x corr_rows faces): 1, 2 or 4
 };
 
 // Everything a launcher needs for one batch (device descriptor array, tile

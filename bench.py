@@ -130,8 +130,9 @@ def batched_leg(pkg, args, plain_rate):
     shape = None
     if not args.no_tune:
         shape = gb.autotune(2)
+        shape = gb.launch_shape() + (shape[3],)
         log(f"[bench] batch shape x2_seg={shape[0]} persist={shape[1]} xcd_remap={shape[2]} "
-            f"({shape[3]:.3f} ms/step)")
+            f"corr_rows={shape[3]} ({shape[4]:.3f} ms/step)")
     for _ in range(args.batch_warmup):
         gb.step()
     gb.sync()
@@ -154,7 +155,7 @@ def batched_leg(pkg, args, plain_rate):
                         "off-centre blasts, Roe even / HLLE odd, per-instance CFL dt, "
                         "one HIP graph per step",
             "shape": None if shape is None else {"x2_seg": shape[0], "persist": shape[1],
-                                                 "xcd_remap": shape[2]}}
+                                                 "xcd_remap": shape[2], "corr_rows": shape[3]}}
 
 
 def adjoint_leg(pkg, args):
@@ -311,8 +312,9 @@ def main():
     shape = None
     if not args.no_tune:
         shape = g.autotune(3)
+        shape = g.launch_shape() + (shape[3],)
         log(f"[bench] autotuned shape x2_seg={shape[0]} persist={shape[1]} xcd_remap={shape[2]} "
-            f"({shape[3]:.3f} ms/step)")
+            f"corr_rows={shape[3]} ({shape[4]:.3f} ms/step)")
     for _ in range(args.warmup):
         g.step()
     g.sync()
@@ -356,7 +358,7 @@ def main():
                                "CTU+CT, periodic, CFL 0.4; one step = integrate + new_dt + "
                                "set_bvals", "nx1": n, "nx2": n, "variant": args.variant,
                    "shape": None if shape is None else {"x2_seg": shape[0], "persist": shape[1],
-                                                 "xcd_remap": shape[2]}},
+                                                 "xcd_remap": shape[2], "corr_rows": shape[3]}},
         "roofline": roofline,
         "step_roofline": {"bound": "hbm", "alg_bytes_per_cell_update": ALG_BYTES_PER_CELL,
                           "achieved": step_bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -114,11 +114,19 @@ int athmi_set_shape(athmi_grid *g, int x2_seg, int persist);
 /* Tile order: 1 = each of the 8 XCDs gets one contiguous run of tiles (its L2
  * serves the neighbour rows), 0 = launch order. Results are identical. */
 int athmi_set_tile_order(athmi_grid *g, int xcd_remap);
+/* Rows of a corrector tile (64 x rows faces: 1, 2 or 4; default 4). Results
+ * are identical. */
+int athmi_set_corr_rows(athmi_grid *g, int rows);
+/* The current launch shape (any pointer may be NULL) */
+int athmi_get_shape(const athmi_grid *g, int *x2_seg, int *persist, int *xcd_remap,
+                    int *corr_rows);
 int athmi_use_graph(athmi_grid *g, int on);
 /* Autotuner: time every launch shape (x2 marching depth 16..128 x persistent
- * workgroups per CU 0..8 x tile order) over `steps` main-loop steps from the
- * current state through the step graph, restore the state and clocks, and
- * keep the fastest shape (returned with its ms per step). No reference
+ * workgroups per CU 0..8 x tile order, then the corrector's tile height
+ * 4/2/1 at the best of those) over `steps` main-loop steps from the current
+ * state through the step graph, restore the state and clocks, and keep the
+ * fastest shape (returned with its ms per step; athmi_get_shape gives all of
+ * it). No reference
  * counterpart (the reference launches fixed 32-thread blocks,
  * integrate_2d_cuda.cu:93-193). */
 int athmi_autotune(athmi_grid *g, int steps, int *x2_seg, int *persist, int *xcd_remap,

@@ -22,7 +22,7 @@ def assert_same(a, b, tag):
         assert np.array_equal(x, y), f"{tag} {name}: max diff {np.abs(x - y).max()}"
 
 
-@pytest.mark.parametrize("shape", [None, (64, 2, 0), (16, 0, 1)])
+@pytest.mark.parametrize("shape", [None, (64, 2, 0, 4), (16, 0, 1, 1), (32, 4, 1, 2)])
 def test_batch_instances_equal_plain_runs(shape):
     pkg, E = ens()
     params = E.ensemble_params(seed=7, n=10)
@@ -32,6 +32,8 @@ def test_batch_instances_equal_plain_runs(shape):
     if shape:
         gb.set_shape(*shape[:2])
         gb.set_tile_order(shape[2])
+        gb.set_corr_rows(shape[3])
+        assert gb.launch_shape() == shape
     gb.run()  # issues whole chunks of steps; finished instances idle
     assert all(gb.clock(k)[2] == steps for k in range(gb.n))
     for k, p in enumerate(params):
