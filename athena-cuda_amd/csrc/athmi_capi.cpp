@@ -414,6 +414,8 @@ int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd
   if (!g || steps <= 0) return fail(ATHMI_EINVAL, "athmi_autotune: bad arguments");
   static const int kSeg[] = {16, 32, 64, 128};
   static const int kPersist[] = {0, 1, 2, 4, 8};
+  constexpr size_t kFinalists = 4;
+  constexpr int kFinalRounds = 3;
   HIP_TRY(hipStreamSynchronize(g->stream));
   std::vector<Clock> hc(g->n);
   HIP_TRY(hipMemcpy(hc.data(), g->dclk, sizeof(Clock) * g->n, hipMemcpyDeviceToHost));
@@ -447,7 +449,8 @@ int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = fail(ATHMI_EHIP, "athmi_autotune: event creation failed");
   // time one shape from the saved state; restores state and clocks
-  auto trial = [&](const LaunchShape& sh) -> int {
+  std::vector<std::pair<double, LaunchShape>> timed;
+  auto trial = [&](const LaunchShape& sh, double* out_ms) -> int {
     g->bl.shape = sh;
     int r;
     if ((r = g->refresh_tiles())) return r;
@@ -460,11 +463,18 @@ int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd
     if (hipEventRecord(e1, g->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
         hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
       return fail(ATHMI_EHIP, "athmi_autotune: event timing failed");
-    if (ms / steps < best) {
-      best = ms / steps;
+    *out_ms = ms / steps;
+    return copy_state(false);
+  };
+  auto sweep = [&](const LaunchShape& sh) -> int {
+    double ms = 0.0;
+    const int r = trial(sh, &ms);
+    if (!r) timed.emplace_back(ms, sh);
+    if (!r && ms < best) {
+      best = ms;
       best_shape = sh;
     }
-    return copy_state(false);
+    return r;
   };
   // tile order x x2 marching depth x persistent occupancy, then the
   // corrector's tile height at the best of those
@@ -475,14 +485,36 @@ int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd
         sh.x2_seg = kSeg[a];
         sh.persist = kPersist[b];
         sh.xcd_remap = c;
-        rc = trial(sh);
+        rc = sweep(sh);
       }
   const LaunchShape base = best_shape;
   for (int rows : {4, 2, 1}) {
     if (rc || rows == base.corr_rows) continue;
     LaunchShape sh = base;
     sh.corr_rows = rows;
-    rc = trial(sh);
+    rc = sweep(sh);
+  }
+  // single short trials differ by about 1% from run to run: the fastest few
+  // shapes run again, in turn, and the lowest total time wins
+  if (!rc && timed.size() > 1) {
+    std::stable_sort(timed.begin(), timed.end(),
+                     [](const std::pair<double, LaunchShape>& x,
+                        const std::pair<double, LaunchShape>& y) { return x.first < y.first; });
+    const size_t nf = std::min<size_t>(kFinalists, timed.size());
+    std::vector<double> total(nf, 0.0);
+    for (int round = 0; !rc && round < kFinalRounds; round++)
+      for (size_t f = 0; !rc && f < nf; f++) {
+        double ms = 0.0;
+        rc = trial(timed[f].second, &ms);
+        total[f] += ms;
+      }
+    if (!rc) {
+      size_t w = 0;
+      for (size_t f = 1; f < nf; f++)
+        if (total[f] < total[w]) w = f;
+      best_shape = timed[w].second;
+      best = total[w] / kFinalRounds;
+    }
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);

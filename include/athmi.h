@@ -124,11 +124,11 @@ int athmi_use_graph(athmi_grid *g, int on);
 /* Autotuner: time every launch shape (x2 marching depth 16..128 x persistent
  * workgroups per CU 0..8 x tile order, then the corrector's tile height
  * 4/2/1 at the best of those) over `steps` main-loop steps from the current
- * state through the step graph, restore the state and clocks, and keep the
- * fastest shape (returned with its ms per step; athmi_get_shape gives all of
- * it). No reference
- * counterpart (the reference launches fixed 32-thread blocks,
- * integrate_2d_cuda.cu:93-193). */
+ * state through the step graph, restore the state and clocks; the four
+ * fastest shapes then run three more trials each, in turn, and the lowest
+ * total wins (returned with its mean ms per step; athmi_get_shape gives all
+ * of it). No reference counterpart (the reference launches fixed 32-thread
+ * blocks, integrate_2d_cuda.cu:93-193). */
 int athmi_autotune(athmi_grid *g, int steps, int *x2_seg, int *persist, int *xcd_remap,
                    double *ms_per_step);
 /* Time nsteps main-loop steps kernel by kernel (HIP events on the handle's
