@@ -46,7 +46,11 @@ def build(verbose_resources=False):
         for src in ("athmi_capi.cpp", "calib.hip", "adjoint_host.cpp", "adjoint_capi.cpp",
                     "adjoint_kernels.hip", "output.hip", "pgen.hip"):
             o = os.path.join(tmp, src.split(".")[0] + ".o")
-            run([HIPCC, ARCH, *COMMON, *NO_PK, "-ffp-contract=off", "-c",
+            # the PLM reverse's loops over the seven characteristic fields are
+            # fully unrolled (each eigenmatrix entry then folds to a constant
+            # case) although their bodies hold reload fences
+            extra = ["-mllvm", "-pragma-unroll-threshold=200000"] if src == "adjoint_kernels.hip" else []
+            run([HIPCC, ARCH, *COMMON, *NO_PK, "-ffp-contract=off", *extra, "-c",
                  os.path.join(CSRC, src), "-o", o])
             objs.append(o)
         for src in ("host_problems.cpp", "dropin.cpp"):

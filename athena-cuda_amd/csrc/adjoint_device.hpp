@@ -394,15 +394,33 @@ ATHMI_HD constexpr bool prim_Rnz(int n, int m) {
                               : (m != 3);
 }
 
+// After this point values held in memory are read again: a structure kept
+// in LDS is then not cached in registers from one reverse stage to the next.
+ATHMI_HD void reload_fence() { asm volatile("" ::: "memory"); }
+
 // ---------------------------------------------------------------------------
 // lr_states_plm.c:64-248 for one cell i (W[i-1], W[i], W[i+1], Bxc[i]):
 // out_l -> Wl[i+1], out_r -> Wr[i]. Only the stencil and the eigensystem are
 // kept; the forward chain (Steps 2-7) is recomputed, in the reference order,
 // by each reverse stage that needs a part of it, which keeps few values live.
-struct PlmF {
+// PlmCore holds what the reverse reads (it may live in LDS, one per thread);
+// PlmF adds the forward outputs.
+struct PlmCore {
   double wm[7], w0[7], wp[7], bxc, dtodx;
   EsysPrimF E;
-  double out_l[7], out_r[7];
+
+  // the stencil and the eigensystem of W[i] (Step 1)
+  ATHMI_HDM void prepare(const Prim7& Wm, const Prim7& W0, const Prim7& Wp, double bxc_,
+                         double dtodx_, const EosC& eos) {
+    for (int n = 0; n < 7; n++) {
+      wm[n] = arr(Wm)[n];
+      w0[n] = arr(W0)[n];
+      wp[n] = arr(Wp)[n];
+    }
+    bxc = bxc_;
+    dtodx = dtodx_;
+    E.run(W0.d, W0.vx, W0.p, bxc, W0.by, W0.bz, eos.gamma);
+  }
 
   // Steps 2-4 for component n: the four characteristic projections and the
   // limited slope
@@ -465,47 +483,6 @@ struct PlmF {
     }
   }
 
-  ATHMI_HDM void run(const Prim7& Wm, const Prim7& W0, const Prim7& Wp, double bxc_,
-                     double dtodx_, const EosC& eos) {
-    for (int n = 0; n < 7; n++) {
-      wm[n] = arr(Wm)[n];
-      w0[n] = arr(W0)[n];
-      wp[n] = arr(Wp)[n];
-    }
-    bxc = bxc_;
-    dtodx = dtodx_;
-    E.run(W0.d, W0.vx, W0.p, bxc, W0.by, W0.bz, eos.gamma);
-    double Wlv[7], Wrv[7], dW[7];
-    chain(nullptr, nullptr, nullptr, Wlv, Wrv, dW);
-    const double qx1 = 0.5 * dmax(E.ev[6], 0.0) * dtodx;  // Step 8 (:181-192)
-    for (int n = 0; n < 7; n++) out_l[n] = Wrv[n] - qx1 * dW[n];
-    const double qx2 = -0.5 * dmin(E.ev[0], 0.0) * dtodx;
-    for (int n = 0; n < 7; n++) out_r[n] = Wlv[n] + qx2 * dW[n];
-    for (int n = 0; n < 7; n++) {  // Step 9 (:201-209)
-      if (E.ev[n] > 0.) {
-        double qa = 0.0;
-        for (int m = 0; m < 7; m++)
-          if (prim_Lnz(n, m)) qa = qa + E.L(n, m) * 0.5 * dtodx * (E.ev[6] - E.ev[n]) * dW[m];
-        for (int m = 0; m < 7; m++)
-          if (prim_Rnz(m, n)) out_l[m] = out_l[m] + qa * E.R(m, n);
-      }
-    }
-    for (int n = 0; n < 7; n++) {  // (:211-224)
-      if (E.ev[n] < 0.) {
-        double qa = 0.0;
-        for (int m = 0; m < 7; m++)
-          if (prim_Lnz(n, m)) qa = qa + E.L(n, m) * 0.5 * dtodx * (E.ev[0] - E.ev[n]) * dW[m];
-        for (int m = 0; m < 7; m++)
-          if (prim_Rnz(m, n)) out_r[m] = out_r[m] + qa * E.R(m, n);
-        qa = 0.0;
-        for (int m = 0; m < 7; m++)
-          if (prim_Lnz(n, m)) qa = qa + E.L(n, m) * 0.5 * dtodx * (E.ev[n] - E.ev[6]) * dW[m];
-        for (int m = 0; m < 7; m++)
-          if (prim_Rnz(m, n)) out_l[m] = out_l[m] - qa * E.R(m, n);
-      }
-    }
-  }
-
   // Adjoint: out_lb, out_rb -> (+=) wmb, w0b, wpb, bxcb. The adjoints of the
   // eigenmatrix entries are folded into the eigensystem's intermediates as
   // they arise (EsysPrimF::addR / addL).
@@ -524,6 +501,7 @@ struct PlmF {
       // Step 9
 #pragma unroll
       for (int n = 0; n < 7; n++) {
+        reload_fence();
         if (!(E.ev[n] > 0.) && !(E.ev[n] < 0.)) continue;
         double LdW = 0.0;
 #pragma unroll
@@ -572,6 +550,7 @@ struct PlmF {
         }
       }
       // Step 8
+      reload_fence();
       const double qx1 = 0.5 * dmax(E.ev[6], 0.0) * dtodx;
       const double qx2 = -0.5 * dmin(E.ev[0], 0.0) * dtodx;
       double qx1b = 0.0, qx2b = 0.0;
@@ -587,6 +566,7 @@ struct PlmF {
     // Step 7: clamps, then Step 5 (dWm = R da)
     {
       double da[7], Wlv0[7], Wrv0[7];
+      reload_fence();
       chain(da, Wlv0, Wrv0, nullptr, nullptr, nullptr);
       double dWmb[7];
 #pragma unroll
@@ -675,6 +655,7 @@ struct PlmF {
     }
     // eigensystem of W[i] (d, vx, p, Bxc, by, bz)
     double inb[6] = {0, 0, 0, 0, 0, 0};
+    reload_fence();
     E.back_acc(A, evb, inb);
     w0b[0] += inb[0];
     w0b[1] += inb[1];
@@ -682,6 +663,44 @@ struct PlmF {
     bxcb += inb[3];
     w0b[5] += inb[4];
     w0b[6] += inb[5];
+  }
+};
+
+struct PlmF : PlmCore {
+  double out_l[7], out_r[7];
+
+  ATHMI_HDM void run(const Prim7& Wm, const Prim7& W0, const Prim7& Wp, double bxc_,
+                     double dtodx_, const EosC& eos) {
+    prepare(Wm, W0, Wp, bxc_, dtodx_, eos);
+    double Wlv[7], Wrv[7], dW[7];
+    chain(nullptr, nullptr, nullptr, Wlv, Wrv, dW);
+    const double qx1 = 0.5 * dmax(E.ev[6], 0.0) * dtodx;  // Step 8 (:181-192)
+    for (int n = 0; n < 7; n++) out_l[n] = Wrv[n] - qx1 * dW[n];
+    const double qx2 = -0.5 * dmin(E.ev[0], 0.0) * dtodx;
+    for (int n = 0; n < 7; n++) out_r[n] = Wlv[n] + qx2 * dW[n];
+    for (int n = 0; n < 7; n++) {  // Step 9 (:201-209)
+      if (E.ev[n] > 0.) {
+        double qa = 0.0;
+        for (int m = 0; m < 7; m++)
+          if (prim_Lnz(n, m)) qa = qa + E.L(n, m) * 0.5 * dtodx * (E.ev[6] - E.ev[n]) * dW[m];
+        for (int m = 0; m < 7; m++)
+          if (prim_Rnz(m, n)) out_l[m] = out_l[m] + qa * E.R(m, n);
+      }
+    }
+    for (int n = 0; n < 7; n++) {  // (:211-224)
+      if (E.ev[n] < 0.) {
+        double qa = 0.0;
+        for (int m = 0; m < 7; m++)
+          if (prim_Lnz(n, m)) qa = qa + E.L(n, m) * 0.5 * dtodx * (E.ev[0] - E.ev[n]) * dW[m];
+        for (int m = 0; m < 7; m++)
+          if (prim_Rnz(m, n)) out_r[m] = out_r[m] + qa * E.R(m, n);
+        qa = 0.0;
+        for (int m = 0; m < 7; m++)
+          if (prim_Lnz(n, m)) qa = qa + E.L(n, m) * 0.5 * dtodx * (E.ev[n] - E.ev[6]) * dW[m];
+        for (int m = 0; m < 7; m++)
+          if (prim_Rnz(m, n)) out_l[m] = out_l[m] - qa * E.R(m, n);
+      }
+    }
   }
 };
 

@@ -603,8 +603,11 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_plm(AdjArgs a) {
   const uint32_t kc[3] = {k - stride, k, k + stride};
   Prim7 W[3];
   for (int c = 0; c < 3; c++) W[c] = cons_to_prim(cons_at(kc[c]), st[qx * ps + kc[c]], eos);
-  PlmF P;
-  P.run(W[0], W[1], W[2], st[qx * ps + k], a.dt / (X1 ? g.dx1 : g.dx2), eos);
+  // the stencil and eigensystem sit in LDS, one per thread, and each reverse
+  // stage reads them again: the reverse then fits in registers without spills
+  __shared__ PlmCore slot[kTileX * kTileY];
+  PlmCore& P = slot[threadIdx.y * kTileX + threadIdx.x];
+  P.prepare(W[0], W[1], W[2], st[qx * ps + k], a.dt / (X1 ? g.dx1 : g.dx2), eos);
   double wb[3][7];
   for (int c = 0; c < 3; c++)
     for (int n = 0; n < 7; n++) wb[c][n] = 0.0;
