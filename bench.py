@@ -209,7 +209,7 @@ def adjoint_leg(pkg, args):
     t_bwd = time.perf_counter() - t0
     ratio = (t_fwd + t_bwd) / t_plain
     log(f"[bench] adjoint: plain {1e3 * t_plain:.1f} ms, forward+ckpt {1e3 * t_fwd:.1f} ms, "
-        f"reverse {1e3 * t_bwd:.1f} ms, ratio {ratio:.2f}")
+        f"reverse {1e3 * t_bwd:.1f} ms, ratio {ratio:.2f} (reverse alone {t_bwd / t_plain:.2f})")
 
     def fd_check(state, tg, handle, dirs, eps=1e-6):
         """max relative |FD - adjoint| over the directions (central, eps)."""
@@ -250,7 +250,8 @@ def adjoint_leg(pkg, args):
     log(f"[bench] adjoint FD (smooth state, all fields, {len(dirs)}): max rel {rel_sm:.2e} "
         f"(eps 1e-6), {rel_sm8:.2e} (eps 1e-8)")
     h2.close()
-    return {"cost_ratio": ratio, "plain_forward_ms": 1e3 * t_plain,
+    return {"cost_ratio": ratio, "reverse_to_plain": t_bwd / t_plain,
+            "plain_forward_ms": 1e3 * t_plain,
             "forward_with_checkpoints_ms": 1e3 * t_fwd, "reverse_ms": 1e3 * t_bwd,
             "workload": f"{n}x{n} periodic, rho = 1 + 16 Fourier modes, P = 1, v = 0, "
                         f"B = (0.5, 0.2, 0), dt = {dt}, N = {N}, checkpoint every {every}",
