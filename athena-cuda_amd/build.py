@@ -46,10 +46,15 @@ def build(verbose_resources=False):
         for src in ("athmi_capi.cpp", "calib.hip", "adjoint_host.cpp", "adjoint_capi.cpp",
                     "adjoint_kernels.hip", "output.hip", "pgen.hip"):
             o = os.path.join(tmp, src.split(".")[0] + ".o")
-            # the PLM reverse's loops over the seven characteristic fields are
-            # fully unrolled (each eigenmatrix entry then folds to a constant
-            # case) although their bodies hold reload fences
-            extra = ["-mllvm", "-pragma-unroll-threshold=200000"] if src == "adjoint_kernels.hip" else []
+            # adjoint kernels: the PLM reverse's loops over the seven
+            # characteristic fields are fully unrolled (each eigenmatrix entry
+            # then folds to a constant case) although their bodies hold reload
+            # fences; the scheduler's own register-pressure trackers cut the
+            # spills of the Roe and PLM reverses (one wave per SIMD) by 7-35%
+            # and their time by about 10% (measured at 512x512)
+            extra = (["-mllvm", "-pragma-unroll-threshold=200000",
+                      "-mllvm", "-amdgpu-use-amdgpu-trackers"]
+                     if src == "adjoint_kernels.hip" else [])
             run([HIPCC, ARCH, *COMMON, *NO_PK, "-ffp-contract=off", *extra, "-c",
                  os.path.join(CSRC, src), "-o", o])
             objs.append(o)
