@@ -759,6 +759,9 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_divb(const Inst* __restrict_
 
 // ---------------------------------------------------------------------------
 // Host-side launchers of this variant
+// resident x1-predictor tiles per CU at one wave per SIMD
+constexpr int kX1PerCu = 4 * 64 / kX1Block;
+// persistent grids: persist x per_cu workgroups per CU (persist 0: one per tile)
 static int grid_of(const BatchLaunch& b, int kind, int per_cu) {
   const int n = b.ntiles[kind];
   if (b.shape.persist <= 0) return n;
@@ -789,11 +792,11 @@ hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
                          kernel == SK_BEGIN_STEP ? 0 : 1);
       break;
     case SK_PRED_X1:
-      hipLaunchKernelGGL(k_pred_x1<false>, dim3(grid_of(b, K_PRED_X1, 4)), dim3(kX1Block), 0, s,
+      hipLaunchKernelGGL(k_pred_x1<false>, dim3(grid_of(b, K_PRED_X1, kX1PerCu)), dim3(kX1Block), 0, s,
                          b.insts, b.ninst, b.ntiles[K_PRED_X1], b.shape.xcd_remap);
       break;
     case SK_PRED_X1_SAVE:
-      hipLaunchKernelGGL(k_pred_x1<true>, dim3(grid_of(b, K_PRED_X1, 4)), dim3(kX1Block), 0, s,
+      hipLaunchKernelGGL(k_pred_x1<true>, dim3(grid_of(b, K_PRED_X1, kX1PerCu)), dim3(kX1Block), 0, s,
                          b.insts, b.ninst, b.ntiles[K_PRED_X1], b.shape.xcd_remap);
       break;
     case SK_PRED_X2:
