@@ -950,6 +950,14 @@ ATHMI_HD Cons7 riemann_reload(int kind, const Cons7& Ul, const Cons7& Ur, const 
 }
 
 
+// Riemann solver fixed at compile time (a kernel then holds one solver only)
+template <int KIND, class Reload>
+ATHMI_HD Cons7 riemann_of(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl, const Prim7& Wr,
+                          double Bxi, const EosC& eos, Reload&& reload) {
+  if constexpr (KIND == kFluxHlle) return flux_hlle(Ul, Ur, Wl, Wr, Bxi, eos);
+  else return flux_roe_reload(Ul, Ur, Wl, Wr, Bxi, eos, reload);
+}
+
 ATHMI_HD Cons7 riemann(int kind, const Cons7& Ul, const Cons7& Ur, const Prim7& Wl,
                        const Prim7& Wr, double Bxi, const EosC& eos) {
   return kind == kFluxHlle ? flux_hlle(Ul, Ur, Wl, Wr, Bxi, eos)

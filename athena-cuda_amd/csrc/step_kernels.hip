@@ -362,14 +362,17 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_corner(const Inst* __restric
 // PART 1: the x1 faces (Steps 5, 8b); PART 2: the x2 faces and Step 7 (6, 8c, 7).
 // Split so that each thread evaluates one Riemann flux: the loads of the second
 // face no longer compete for registers with the first face's flux.
-template <int PART, int TY>  // TY: rows of a tile of kTileX x TY faces
-__global__ void __launch_bounds__(kTileX* TY) k_corr(const Inst* __restrict__ insts,
-                                                         int ninst, int ntiles, int remap) {
+// FLUX: the Riemann solver; a launch takes only the tiles of instances that use
+// it. With one solver per kernel (the Roe one keeps its HLLE fallback) the
+// corrector fits two waves per SIMD.
+template <int PART, int TY, int FLUX>  // TY: rows of a tile of kTileX x TY faces
+__global__ void __launch_bounds__(kTileX* TY, 2) k_corr(const Inst* __restrict__ insts,
+                                                            int ninst, int ntiles, int remap) {
   for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
     const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, K_CORR, t)];
     const Clock* clk = g.clk;
-    if (!clk->active) continue;
+    if (!clk->active || g.flux != FLUX) continue;
     const int local = t - g.tile0[K_CORR];
     const int ntx = dcdiv(g.nx1 + 3, kTileX);
     const int ty = local / ntx, tx = local - ty * ntx;
@@ -441,7 +444,7 @@ __global__ void __launch_bounds__(kTileX* TY) k_corr(const Inst* __restrict__ in
       Prim7 wl, wr;
       double bx;
       build(L, R, wl, wr, bx);
-      put7(sc + S_F1C * ps, ps, k, riemann_reload(g.flux, L, R, wl, wr, bx, eos, build));
+      put7(sc + S_F1C * ps, ps, k, riemann_of<FLUX>(L, R, wl, wr, bx, eos, build));
     }
     if (PART == 1) continue;
     if (j >= g.js) {
@@ -492,7 +495,7 @@ __global__ void __launch_bounds__(kTileX* TY) k_corr(const Inst* __restrict__ in
       Prim7 wl, wr;
       double bx;
       build(L, R, wl, wr, bx);
-      put7(sc + S_F2C * ps, ps, k, riemann_reload(g.flux, L, R, wl, wr, bx, eos, build));
+      put7(sc + S_F2C * ps, ps, k, riemann_of<FLUX>(L, R, wl, wr, bx, eos, build));
     }
     // Step 7
     const double dhalf = LD(st + F_D * ps, k) - hdtodx1 * (LD(F1, k + 1) - LD(F1, k)) -
@@ -761,6 +764,21 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_divb(const Inst* __restrict_
 // Host-side launchers of this variant
 // resident x1-predictor tiles per CU at one wave per SIMD
 constexpr int kX1PerCu = 4 * 64 / kX1Block;
+typedef void (*CorrKernel)(const Inst*, int, int, int);
+// the corrector instantiation for a part (1: x1 faces, 2: x2 faces and Step
+// 7), tile height and Riemann solver
+template <int PART>
+static CorrKernel corr_kernel_part(int ty, int flux) {
+  if (flux == kFluxHlle)
+    return ty == 1 ? k_corr<PART, 1, kFluxHlle> : ty == 2 ? k_corr<PART, 2, kFluxHlle>
+                                                            : k_corr<PART, 4, kFluxHlle>;
+  return ty == 1 ? k_corr<PART, 1, kFluxRoe> : ty == 2 ? k_corr<PART, 2, kFluxRoe>
+                                                         : k_corr<PART, 4, kFluxRoe>;
+}
+static CorrKernel corr_kernel(int part, int ty, int flux) {
+  return part == 1 ? corr_kernel_part<1>(ty, flux) : corr_kernel_part<2>(ty, flux);
+}
+
 // persistent grids: persist x per_cu workgroups per CU (persist 0: one per tile)
 static int grid_of(const BatchLaunch& b, int kind, int per_cu) {
   const int n = b.ntiles[kind];
@@ -815,19 +833,16 @@ hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
       break;
     case SK_CORR:
     case SK_CORR2: {
-      // the same threads in flight per CU whatever the tile height
+      // the same threads in flight per CU whatever the tile height; one launch
+      // per Riemann solver in use
       const int ty = b.shape.corr_rows;
       const dim3 grid(grid_of(b, K_CORR, 4 * (kTileY / ty))), blk(kTileX, ty);
       const bool x1 = (kernel == SK_CORR);
-      if (ty == 1) {
-        if (x1) hipLaunchKernelGGL((k_corr<1, 1>), grid, blk, 0, s, b.insts, b.ninst, b.ntiles[K_CORR], b.shape.xcd_remap);
-        else hipLaunchKernelGGL((k_corr<2, 1>), grid, blk, 0, s, b.insts, b.ninst, b.ntiles[K_CORR], b.shape.xcd_remap);
-      } else if (ty == 2) {
-        if (x1) hipLaunchKernelGGL((k_corr<1, 2>), grid, blk, 0, s, b.insts, b.ninst, b.ntiles[K_CORR], b.shape.xcd_remap);
-        else hipLaunchKernelGGL((k_corr<2, 2>), grid, blk, 0, s, b.insts, b.ninst, b.ntiles[K_CORR], b.shape.xcd_remap);
-      } else {
-        if (x1) hipLaunchKernelGGL((k_corr<1, 4>), grid, blk, 0, s, b.insts, b.ninst, b.ntiles[K_CORR], b.shape.xcd_remap);
-        else hipLaunchKernelGGL((k_corr<2, 4>), grid, blk, 0, s, b.insts, b.ninst, b.ntiles[K_CORR], b.shape.xcd_remap);
+      for (int f = 0; f < 2; f++) {
+        if (!(b.flux_mask & (1 << f))) continue;
+        const CorrKernel fn = corr_kernel(x1 ? 1 : 2, ty, f);
+        hipLaunchKernelGGL(fn, grid, blk, 0, s, b.insts, b.ninst, b.ntiles[K_CORR],
+                           b.shape.xcd_remap);
       }
       break;
     }

@@ -116,6 +116,7 @@ struct BatchLaunch {
   int ntiles[NKINDS];
   LaunchShape shape;
   int ncu;
+  int flux_mask;  // bit f set: some instance uses Riemann solver f (kFluxRoe, kFluxHlle)
 };
 
 // Kernels of the pipeline, launched one at a time by athmi_<variant>_launch
