@@ -121,9 +121,12 @@ __device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double 
 // thread converts one cell to primitives; neighbours are exchanged in LDS for
 // the characteristic PLM (lr_states_plm.c:95-245); each face then gets the
 // 1/2 dt MHD source on By, prim->cons and the Riemann flux.
-template <bool SAVE>  // SAVE: also store the primitive face states (Inst::prims)
-__global__ void __launch_bounds__(kX1Block) k_pred_x1(const Inst* __restrict__ insts, int ninst,
-                                                      int ntiles, int remap) {
+// SAVE: also store the primitive face states (Inst::prims). FLUX: the Riemann
+// solver; a launch takes only the tiles of instances that use it, and with one
+// solver per kernel the predictor fits two waves per SIMD.
+template <bool SAVE, int FLUX>
+__global__ void __launch_bounds__(kX1Block, 2) k_pred_x1(const Inst* __restrict__ insts,
+                                                         int ninst, int ntiles, int remap) {
   __shared__ double sw[7][kX1Block];
   __shared__ double swl[7][kX1Block];
   __shared__ double swr[7][kX1Block];
@@ -132,7 +135,7 @@ __global__ void __launch_bounds__(kX1Block) k_pred_x1(const Inst* __restrict__ i
     const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, K_PRED_X1, t)];
     const Clock* clk = g.clk;
-    if (!clk->active) continue;
+    if (!clk->active || g.flux != FLUX) continue;
     const int local = t - g.tile0[K_PRED_X1];
     const int nseg = dcdiv(g.nx1 + 4, kX1Faces);
     const int row = local / nseg, seg = local - row * nseg;
@@ -195,7 +198,7 @@ __global__ void __launch_bounds__(kX1Block) k_pred_x1(const Inst* __restrict__ i
       }
       put7(sc + S_UL1 * ps, ps, k, ul);
       put7(sc + S_UR1 * ps, ps, k, ur);
-      put7(sc + S_F1 * ps, ps, k, riemann_reload(g.flux, ul, ur, wl, wr, b0, eos, build));
+      put7(sc + S_F1 * ps, ps, k, riemann_of<FLUX>(ul, ur, wl, wr, b0, eos, build));
     }
     __syncthreads();
   }
@@ -206,7 +209,9 @@ __global__ void __launch_bounds__(kX1Block) k_pred_x1(const Inst* __restrict__ i
 // order (d, M2, M3, M1, E, B3c, B1c) with Bx = B2c. Lanes own adjacent
 // columns (coalesced rows); each lane marches down a segment of faces keeping
 // the three-cell PLM stencil and the pending left state in registers.
-template <bool SAVE>
+// (SAVE and FLUX as for the x1 predictor; the marching stencil keeps this one
+// at one wave per SIMD)
+template <bool SAVE, int FLUX>
 __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ insts, int ninst,
                                                       int ntiles, int seg_len, int remap) {
   __shared__ double sxl[7][kX2Block];
@@ -216,7 +221,7 @@ __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ i
     const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, K_PRED_X2, t)];
     const Clock* clk = g.clk;
-    if (!clk->active) continue;
+    if (!clk->active || g.flux != FLUX) continue;
     const int local = t - g.tile0[K_PRED_X2];
     const int ncb = dcdiv(g.nx1 + 5, kX2Block);
     const int sg = local / ncb, cb = local - sg * ncb;
@@ -294,7 +299,7 @@ __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ i
       }
       put7(sc + S_UL2 * ps, ps, k, ul);
       put7(sc + S_UR2 * ps, ps, k, ur);
-      put7(sc + S_F2 * ps, ps, k, riemann_reload(g.flux, ul, ur, wl, wr, b0, eos, build));
+      put7(sc + S_F2 * ps, ps, k, riemann_of<FLUX>(ul, ur, wl, wr, b0, eos, build));
       wl_cur = wl_next;
     }
   }
@@ -762,8 +767,19 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_divb(const Inst* __restrict_
 
 // ---------------------------------------------------------------------------
 // Host-side launchers of this variant
-// resident x1-predictor tiles per CU at one wave per SIMD
-constexpr int kX1PerCu = 4 * 64 / kX1Block;
+// resident x1-predictor tiles per CU at two waves per SIMD
+constexpr int kX1PerCu = 2 * 4 * 64 / kX1Block;
+
+typedef void (*PredX1Kernel)(const Inst*, int, int, int);
+typedef void (*PredX2Kernel)(const Inst*, int, int, int, int);
+static PredX1Kernel pred_x1_kernel(bool save, int flux) {
+  if (flux == kFluxHlle) return save ? k_pred_x1<true, kFluxHlle> : k_pred_x1<false, kFluxHlle>;
+  return save ? k_pred_x1<true, kFluxRoe> : k_pred_x1<false, kFluxRoe>;
+}
+static PredX2Kernel pred_x2_kernel(bool save, int flux) {
+  if (flux == kFluxHlle) return save ? k_pred_x2<true, kFluxHlle> : k_pred_x2<false, kFluxHlle>;
+  return save ? k_pred_x2<true, kFluxRoe> : k_pred_x2<false, kFluxRoe>;
+}
 typedef void (*CorrKernel)(const Inst*, int, int, int);
 // the corrector instantiation for a part (1: x1 faces, 2: x2 faces and Step
 // 7), tile height and Riemann solver
@@ -810,22 +826,22 @@ hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
                          kernel == SK_BEGIN_STEP ? 0 : 1);
       break;
     case SK_PRED_X1:
-      hipLaunchKernelGGL(k_pred_x1<false>, dim3(grid_of(b, K_PRED_X1, kX1PerCu)), dim3(kX1Block), 0, s,
-                         b.insts, b.ninst, b.ntiles[K_PRED_X1], b.shape.xcd_remap);
-      break;
     case SK_PRED_X1_SAVE:
-      hipLaunchKernelGGL(k_pred_x1<true>, dim3(grid_of(b, K_PRED_X1, kX1PerCu)), dim3(kX1Block), 0, s,
-                         b.insts, b.ninst, b.ntiles[K_PRED_X1], b.shape.xcd_remap);
+      for (int f = 0; f < 2; f++) {
+        if (!(b.flux_mask & (1 << f))) continue;
+        const PredX1Kernel fn = pred_x1_kernel(kernel == SK_PRED_X1_SAVE, f);
+        hipLaunchKernelGGL(fn, dim3(grid_of(b, K_PRED_X1, kX1PerCu)), dim3(kX1Block), 0, s,
+                           b.insts, b.ninst, b.ntiles[K_PRED_X1], b.shape.xcd_remap);
+      }
       break;
     case SK_PRED_X2:
-      hipLaunchKernelGGL(k_pred_x2<false>, dim3(grid_of(b, K_PRED_X2, 16)), dim3(kX2Block), 0, s,
-                         b.insts, b.ninst, b.ntiles[K_PRED_X2], b.shape.x2_seg,
-                         b.shape.xcd_remap);
-      break;
     case SK_PRED_X2_SAVE:
-      hipLaunchKernelGGL(k_pred_x2<true>, dim3(grid_of(b, K_PRED_X2, 16)), dim3(kX2Block), 0, s,
-                         b.insts, b.ninst, b.ntiles[K_PRED_X2], b.shape.x2_seg,
-                         b.shape.xcd_remap);
+      for (int f = 0; f < 2; f++) {
+        if (!(b.flux_mask & (1 << f))) continue;
+        const PredX2Kernel fn = pred_x2_kernel(kernel == SK_PRED_X2_SAVE, f);
+        hipLaunchKernelGGL(fn, dim3(grid_of(b, K_PRED_X2, 16)), dim3(kX2Block), 0, s, b.insts,
+                           b.ninst, b.ntiles[K_PRED_X2], b.shape.x2_seg, b.shape.xcd_remap);
+      }
       break;
     case SK_CORNER_P:
       hipLaunchKernelGGL(k_corner<true>, dim3(grid_of(b, K_CORNER_P, 8)), b2, 0, s, b.insts,
