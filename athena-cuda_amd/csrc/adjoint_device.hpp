@@ -1583,6 +1583,18 @@ ATHMI_HD int flux_roe_b(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl, const
   return 0;
 }
 
+// the solver fixed at compile time: a kernel then holds one reverse only (the
+// Roe reverse keeps its HLLE fallback)
+template <int KIND>
+ATHMI_HD void riemann_b_of(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl, const Prim7& Wr,
+                           double Bxi, const EosC& eos, const Cons7& Fb, Cons7& Ulb, Cons7& Urb,
+                           Prim7& Wlb, Prim7& Wrb, double& Bxib) {
+  if constexpr (KIND == kFluxHlle)
+    flux_hlle_b(Ul, Ur, Wl, Wr, Bxi, eos, Fb, Ulb, Urb, Wlb, Wrb, Bxib);
+  else
+    (void)flux_roe_b(Ul, Ur, Wl, Wr, Bxi, eos, Fb, Ulb, Urb, Wlb, Wrb, Bxib);
+}
+
 ATHMI_HD void riemann_b(int kind, const Cons7& Ul, const Cons7& Ur, const Prim7& Wl,
                         const Prim7& Wr, double Bxi, const EosC& eos, const Cons7& Fb,
                         Cons7& Ulb, Cons7& Urb, Prim7& Wlb, Prim7& Wrb, double& Bxib) {

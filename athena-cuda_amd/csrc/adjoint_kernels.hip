@@ -295,7 +295,7 @@ __device__ __forceinline__ CorrCtx corr_ctx(const AdjArgs& a) {
 
 // Step 8b (X1: x1 faces [js-1,je+1] x [is,ie+1], :438-447) or 8c (x2 faces
 // [js,je+1] x [is-1,ie+1], :455-464); the two are independent
-template <bool X1>
+template <bool X1, int FLUX>  // FLUX: the grid's Riemann solver
 __global__ void __launch_bounds__(kTileX* kTileY) ak_riemann_c(AdjArgs a) {
   const Inst& g = a.g;
   int j, i;
@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_riemann_c(AdjArgs a) {
     Cons7 Lb = zero_cons(), Rb = zero_cons();
     Prim7 Wlb = zero_prim(), Wrb = zero_prim();
     double bxb = 0.0;
-    riemann_b(g.flux, L, R, wl, wr, bx, eos, ld7(a.ab + A_F1C * ps, ps, k), Lb, Rb, Wlb, Wrb, bxb);
+    riemann_b_of<FLUX>(L, R, wl, wr, bx, eos, ld7(a.ab + A_F1C * ps, ps, k), Lb, Rb, Wlb, Wrb, bxb);
     cons_to_prim_b(L, bx, eos, Wlb, Lb, bxb);
     cons_to_prim_b(R, bx, eos, Wrb, Rb, bxb);
     st7(a.ab + A_UL1 * ps, ps, k, Lb);
@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_riemann_c(AdjArgs a) {
     Cons7 Lb = zero_cons(), Rb = zero_cons();
     Prim7 Wlb = zero_prim(), Wrb = zero_prim();
     double bxb = 0.0;
-    riemann_b(g.flux, L, R, wl, wr, bx, eos, ld7(a.ab + A_F2C * ps, ps, k), Lb, Rb, Wlb, Wrb, bxb);
+    riemann_b_of<FLUX>(L, R, wl, wr, bx, eos, ld7(a.ab + A_F2C * ps, ps, k), Lb, Rb, Wlb, Wrb, bxb);
     cons_to_prim_b(L, bx, eos, Wlb, Lb, bxb);
     cons_to_prim_b(R, bx, eos, Wrb, Rb, bxb);
     st7(a.ab + A_UL2 * ps, ps, k, Lb);
@@ -510,7 +510,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_emf_cc(AdjArgs a) {
 // (By for x1 faces, Bz for x2 faces), from the face states the forward pass
 // stored. X1: faces [js-2,je+2] x [is-1,ie+2]; x2: [js-1,je+2] x [is-2,ie+2].
 // Writes the adjoints of the PLM outputs Wl, Wr (the source only adds to them).
-template <bool X1>
+template <bool X1, int FLUX>  // FLUX: the grid's Riemann solver
 __global__ void __launch_bounds__(kTileX* kTileY) ak_pred(AdjArgs a) {
   const Inst& g = a.g;
   int j, i;
@@ -534,7 +534,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_pred(AdjArgs a) {
   {
     const Prim7 wl = ld7p(a.pw + (X1 ? P_WL1 : P_WL2) * ps, ps, k);
     const Prim7 wr = ld7p(a.pw + (X1 ? P_WR1 : P_WR2) * ps, ps, k);
-    riemann_b(g.flux, ld7(a.sc + (X1 ? S_UL1 : S_UL2) * ps, ps, k),
+    riemann_b_of<FLUX>(ld7(a.sc + (X1 ? S_UL1 : S_UL2) * ps, ps, k),
               ld7(a.sc + (X1 ? S_UR1 : S_UR2) * ps, ps, k), wl, wr, b0, eos,
               ld7(a.ab + (X1 ? A_F1 : A_F2) * ps, ps, k), Ulb, Urb, wlb, wrb, b0b);
     prim_to_cons_b(wl, b0, eos, Ulb, wlb, b0b);
@@ -687,10 +687,20 @@ extern "C" hipError_t athmi_adjoint_launch(const AdjArgs* ap, int kernel, hipStr
       hipLaunchKernelGGL(ak_corner<false>, grid2(g.js, g.je + 1, g.is, g.ie + 1), b2, 0, s, a);
       break;
     case AK_RIEMANN_C1:
-      hipLaunchKernelGGL(ak_riemann_c<true>, grid2(g.js - 1, g.je + 1, g.is, g.ie + 1), b2, 0, s, a);
+      if (g.flux == kFluxHlle)
+        hipLaunchKernelGGL((ak_riemann_c<true, kFluxHlle>), grid2(g.js - 1, g.je + 1, g.is, g.ie + 1),
+                           b2, 0, s, a);
+      else
+        hipLaunchKernelGGL((ak_riemann_c<true, kFluxRoe>), grid2(g.js - 1, g.je + 1, g.is, g.ie + 1),
+                           b2, 0, s, a);
       break;
     case AK_RIEMANN_C2:
-      hipLaunchKernelGGL(ak_riemann_c<false>, grid2(g.js, g.je + 1, g.is - 1, g.ie + 1), b2, 0, s, a);
+      if (g.flux == kFluxHlle)
+        hipLaunchKernelGGL((ak_riemann_c<false, kFluxHlle>), grid2(g.js, g.je + 1, g.is - 1, g.ie + 1),
+                           b2, 0, s, a);
+      else
+        hipLaunchKernelGGL((ak_riemann_c<false, kFluxRoe>), grid2(g.js, g.je + 1, g.is - 1, g.ie + 1),
+                           b2, 0, s, a);
       break;
     case AK_HALF:
       hipLaunchKernelGGL(ak_half, grid2(g.js - 1, g.je + 1, g.is - 1, g.ie + 1), b2, 0, s, a);
@@ -705,13 +715,23 @@ extern "C" hipError_t athmi_adjoint_launch(const AdjArgs* ap, int kernel, hipStr
       hipLaunchKernelGGL(ak_emf_cc, grid2(g.js - 2, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);
       break;
     case AK_PRED_X1:
-      hipLaunchKernelGGL(ak_pred<true>, grid2(g.js - 2, g.je + 2, g.is - 1, g.ie + 2), b2, 0, s, a);
+      if (g.flux == kFluxHlle)
+        hipLaunchKernelGGL((ak_pred<true, kFluxHlle>), grid2(g.js - 2, g.je + 2, g.is - 1, g.ie + 2),
+                           b2, 0, s, a);
+      else
+        hipLaunchKernelGGL((ak_pred<true, kFluxRoe>), grid2(g.js - 2, g.je + 2, g.is - 1, g.ie + 2),
+                           b2, 0, s, a);
       break;
     case AK_PLM_X1:
       hipLaunchKernelGGL(ak_plm<true>, grid2(g.js - 2, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);
       break;
     case AK_PRED_X2:
-      hipLaunchKernelGGL(ak_pred<false>, grid2(g.js - 1, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);
+      if (g.flux == kFluxHlle)
+        hipLaunchKernelGGL((ak_pred<false, kFluxHlle>), grid2(g.js - 1, g.je + 2, g.is - 2, g.ie + 2),
+                           b2, 0, s, a);
+      else
+        hipLaunchKernelGGL((ak_pred<false, kFluxRoe>), grid2(g.js - 1, g.je + 2, g.is - 2, g.ie + 2),
+                           b2, 0, s, a);
       break;
     case AK_PLM_X2:
       hipLaunchKernelGGL(ak_plm<false>, grid2(g.js - 2, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);
