@@ -74,8 +74,9 @@ enum KernelKind {
 
 // Work decomposition (tile sizes) shared by host and device
 // threads of a predictor-x1 tile (one row segment): two waves, so that the
-// two halves of a CU's four waves pass their block barriers independently
-// (measured at 4096^2: 2.56 ms with 256 threads, 2.45 with 64, 2.40 with 128)
+// workgroups resident on a CU pass their block barriers independently (at one
+// wave per SIMD, 4096^2: 2.56 ms with 256 threads, 2.45 with 64, 2.40 with
+// 128; at two waves per SIMD the three are within 1%)
 constexpr int kX1Block = 128;
 constexpr int kX1Faces = kX1Block - 3;      // faces produced per x1 tile
 constexpr int kX2Block = 64;                // columns of a predictor-x2 tile
