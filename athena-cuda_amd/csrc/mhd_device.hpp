@@ -884,9 +884,20 @@ ATHMI_HD void roe_column_pass(const RoeSpeeds& S, const RoeVecScalars& V, double
 // input states live through the whole Roe evaluation, it asks `reload` to
 // rebuild them (the callers recompute them from memory with the same
 // arithmetic, so the values are identical) and evaluates flux_hlle afresh.
-template <class Reload>
+// Where the half-sum flux f and the intermediate state's start ui wait while
+// the eigen-projection runs: registers by default. A kernel short of registers
+// passes LDS slots instead (the values are written once and read back after a
+// fence, so they hold no registers in between); the arithmetic is unchanged.
+struct RegPark {
+  double v[14];
+  ATHMI_HDM void put(int q, double x) { v[q] = x; }
+  ATHMI_HDM double get(int q) const { return v[q]; }
+  ATHMI_HDM void fence() const {}
+};
+template <class Reload, class Park = RegPark>
 ATHMI_HD Cons7 flux_roe_reload(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl,
-                               const Prim7& Wr, double Bxi, const EosC& eos, Reload&& reload) {
+                               const Prim7& Wr, double Bxi, const EosC& eos, Reload&& reload,
+                               Park park = Park()) {
   const RoeAvg r = roe_average(Ul, Ur, Wl, Wr, Bxi);
   RoeSpeeds S;
   roe_speeds(r.droe, r.v1, r.v2, r.v3, r.h, Bxi, r.b2, r.b3, r.x, r.y, eos, true, S);
@@ -904,6 +915,10 @@ ATHMI_HD Cons7 flux_roe_reload(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl
   const double du[7] = {Ur.d - Ul.d, Ur.mx - Ul.mx, Ur.my - Ul.my, Ur.mz - Ul.mz,
                         Ur.e - Ul.e,  Ur.by - Ul.by, Ur.bz - Ul.bz};
   double ui[7] = {Ul.d, Ul.mx, Ul.my, Ul.mz, Ul.e, Ul.by, Ul.bz};
+  for (int n = 0; n < 7; n++) {
+    park.put(n, f[n]);
+    park.put(7 + n, ui[n]);
+  }
   RoeVecScalars V;
   roe_vec_scalars(S, r.droe, r.v2, r.v3, eos, V);
   double a[7];
@@ -912,6 +927,11 @@ ATHMI_HD Cons7 flux_roe_reload(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl
   double coeff[7];
 #pragma unroll
   for (int m = 0; m < 7; m++) coeff[m] = 0.5 * dmax(fabs(ev[m]), 0.0) * a[m];
+  park.fence();
+  for (int n = 0; n < 7; n++) {
+    f[n] = park.get(n);
+    ui[n] = park.get(7 + n);
+  }
   bool bad = false;
   roe_column_pass<0>(S, V, r.v1, r.v2, r.v3, r.x, eos, a, coeff, ev, Bxi, ui, f, bad);
   roe_column_pass<1>(S, V, r.v1, r.v2, r.v3, r.x, eos, a, coeff, ev, Bxi, ui, f, bad);
@@ -951,11 +971,11 @@ ATHMI_HD Cons7 riemann_reload(int kind, const Cons7& Ul, const Cons7& Ur, const 
 
 
 // Riemann solver fixed at compile time (a kernel then holds one solver only)
-template <int KIND, class Reload>
+template <int KIND, class Reload, class Park = RegPark>
 ATHMI_HD Cons7 riemann_of(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl, const Prim7& Wr,
-                          double Bxi, const EosC& eos, Reload&& reload) {
+                          double Bxi, const EosC& eos, Reload&& reload, Park park = Park()) {
   if constexpr (KIND == kFluxHlle) return flux_hlle(Ul, Ur, Wl, Wr, Bxi, eos);
-  else return flux_roe_reload(Ul, Ur, Wl, Wr, Bxi, eos, reload);
+  else return flux_roe_reload(Ul, Ur, Wl, Wr, Bxi, eos, reload, park);
 }
 
 ATHMI_HD Cons7 riemann(int kind, const Cons7& Ul, const Cons7& Ur, const Prim7& Wl,

@@ -121,6 +121,17 @@ __device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double 
 // thread converts one cell to primitives; neighbours are exchanged in LDS for
 // the characteristic PLM (lr_states_plm.c:95-245); each face then gets the
 // 1/2 dt MHD source on By, prim->cons and the Riemann flux.
+// LDS slots of one thread for flux_roe_reload's parked values (SoA over the
+// workgroup, conflict-free); the corrector uses them to stay within the
+// registers of two waves per SIMD
+struct LdsPark {
+  double* base;  // [14][nthreads]
+  int stride, lane;
+  __device__ void put(int q, double x) const { base[q * stride + lane] = x; }
+  __device__ double get(int q) const { return base[q * stride + lane]; }
+  __device__ void fence() const { compiler_fence(); }
+};
+
 // SAVE: also store the primitive face states (Inst::prims). FLUX: the Riemann
 // solver; a launch takes only the tiles of instances that use it, and with one
 // solver per kernel the predictor fits two waves per SIMD.
@@ -373,6 +384,8 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_corner(const Inst* __restric
 template <int PART, int TY, int FLUX>  // TY: rows of a tile of kTileX x TY faces
 __global__ void __launch_bounds__(kTileX* TY, 2) k_corr(const Inst* __restrict__ insts,
                                                             int ninst, int ntiles, int remap) {
+  __shared__ double park_lds[14 * kTileX * TY];
+  const LdsPark park{park_lds, kTileX * TY, (int)(threadIdx.y * kTileX + threadIdx.x)};
   for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
     const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, K_CORR, t)];
@@ -449,7 +462,7 @@ __global__ void __launch_bounds__(kTileX* TY, 2) k_corr(const Inst* __restrict__
       Prim7 wl, wr;
       double bx;
       build(L, R, wl, wr, bx);
-      put7(sc + S_F1C * ps, ps, k, riemann_of<FLUX>(L, R, wl, wr, bx, eos, build));
+      put7(sc + S_F1C * ps, ps, k, riemann_of<FLUX>(L, R, wl, wr, bx, eos, build, park));
     }
     if (PART == 1) continue;
     if (j >= g.js) {
@@ -500,7 +513,7 @@ __global__ void __launch_bounds__(kTileX* TY, 2) k_corr(const Inst* __restrict__
       Prim7 wl, wr;
       double bx;
       build(L, R, wl, wr, bx);
-      put7(sc + S_F2C * ps, ps, k, riemann_of<FLUX>(L, R, wl, wr, bx, eos, build));
+      put7(sc + S_F2C * ps, ps, k, riemann_of<FLUX>(L, R, wl, wr, bx, eos, build, park));
     }
     // Step 7
     const double dhalf = LD(st + F_D * ps, k) - hdtodx1 * (LD(F1, k + 1) - LD(F1, k)) -
