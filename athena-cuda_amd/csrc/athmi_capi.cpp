@@ -413,7 +413,7 @@ int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd
                    double* ms_per_step) {
   if (!g || steps <= 0) return fail(ATHMI_EINVAL, "athmi_autotune: bad arguments");
   static const int kSeg[] = {16, 32, 64, 128};
-  static const int kPersist[] = {0, 1, 2, 4, 8};
+  static const int kPersist[] = {0, 1, 2, 4, 8, 16, 32};
   constexpr size_t kFinalists = 4;
   constexpr int kFinalRounds = 3;
   HIP_TRY(hipStreamSynchronize(g->stream));

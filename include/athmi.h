@@ -122,7 +122,7 @@ int athmi_get_shape(const athmi_grid *g, int *x2_seg, int *persist, int *xcd_rem
                     int *corr_rows);
 int athmi_use_graph(athmi_grid *g, int on);
 /* Autotuner: time every launch shape (x2 marching depth 16..128 x persistent
- * workgroups per CU 0..8 x tile order, then the corrector's tile height
+ * workgroups per CU 0..32 x tile order, then the corrector's tile height
  * 4/2/1 at the best of those) over `steps` main-loop steps from the current
  * state through the step graph, restore the state and clocks; the four
  * fastest shapes then run three more trials each, in turn, and the lowest
