@@ -286,8 +286,9 @@ def main():
                          "counters calibrate FETCH_SIZE/WRITE_SIZE under rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
-              "one-GPU build (the multi-GPU halo decomposition is design-only).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the library "
+              "runs on one GPU (the multi-GPU halo decomposition is design-only).",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.adjoint:
