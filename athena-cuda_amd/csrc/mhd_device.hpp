@@ -950,38 +950,17 @@ ATHMI_HD Cons7 flux_roe_reload(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl
   return Cons7{f[0], f[1], f[2], f[3], f[4], f[5], f[6]};
 }
 
-ATHMI_HD Cons7 flux_roe(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl, const Prim7& Wr,
-                        double Bxi, const EosC& eos) {
-  return flux_roe_reload(Ul, Ur, Wl, Wr, Bxi, eos,
-                         [&](Cons7& ul, Cons7& ur, Prim7& wl, Prim7& wr, double& bx) {
-                           ul = Ul; ur = Ur; wl = Wl; wr = Wr; bx = Bxi;
-                         });
-}
-
 enum { kFluxRoe = 0, kFluxHlle = 1 };
 
-// Riemann solver of a grid; the Roe solver rebuilds its inputs for the HLLE
-// fallback through `reload` (see flux_roe_reload)
-template <class Reload>
-ATHMI_HD Cons7 riemann_reload(int kind, const Cons7& Ul, const Cons7& Ur, const Prim7& Wl,
-                              const Prim7& Wr, double Bxi, const EosC& eos, Reload&& reload) {
-  if (kind == kFluxHlle) return flux_hlle(Ul, Ur, Wl, Wr, Bxi, eos);
-  return flux_roe_reload(Ul, Ur, Wl, Wr, Bxi, eos, reload);
-}
-
-
-// Riemann solver fixed at compile time (a kernel then holds one solver only)
+// The Riemann solver of a grid, fixed at compile time: a kernel then holds one
+// solver only (a runtime choice made both flux paths share one register
+// allocation). The Roe solver rebuilds its inputs for the HLLE fallback
+// through `reload` and parks f, ui through `park` (see flux_roe_reload).
 template <int KIND, class Reload, class Park = RegPark>
 ATHMI_HD Cons7 riemann_of(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl, const Prim7& Wr,
                           double Bxi, const EosC& eos, Reload&& reload, Park park = Park()) {
   if constexpr (KIND == kFluxHlle) return flux_hlle(Ul, Ur, Wl, Wr, Bxi, eos);
   else return flux_roe_reload(Ul, Ur, Wl, Wr, Bxi, eos, reload, park);
-}
-
-ATHMI_HD Cons7 riemann(int kind, const Cons7& Ul, const Cons7& Ur, const Prim7& Wl,
-                       const Prim7& Wr, double Bxi, const EosC& eos) {
-  return kind == kFluxHlle ? flux_hlle(Ul, Ur, Wl, Wr, Bxi, eos)
-                           : flux_roe(Ul, Ur, Wl, Wr, Bxi, eos);
 }
 
 }  // namespace athmi
