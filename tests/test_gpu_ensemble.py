@@ -45,6 +45,30 @@ def test_batch_instances_equal_plain_runs(shape):
     gb.close()
 
 
+def test_batch_mixing_boundaries_and_solvers_equals_plain_runs():
+    """Instances of one batch with different boundary kinds (set_bvals.c:26-29,
+    per instance in the composed ghost map) and Riemann solvers (each predictor
+    and corrector launch takes only the tiles of its solver): every instance
+    must equal its own plain run, value for value, clocks included."""
+    pkg, E = ens()
+    params = E.ensemble_params(seed=5, n=6)
+    bcs = [(4, 4, 4, 4), (2, 2, 2, 2), (1, 1, 1, 1), (1, 2, 4, 4), (4, 4, 1, 2), (2, 1, 2, 1)]
+    for k, p in enumerate(params):
+        p["bc"] = bcs[k]
+        p["flux"] = "hlle" if k % 3 == 1 else "roe"
+    steps = 30
+    print(f"[gpu] mixed boundaries and solvers, {steps} steps", flush=True)
+    gb = E.build(params, nlim=steps)
+    gb.run()
+    for k, p in enumerate(params):
+        g1 = E.build([p], nlim=steps)
+        g1.run()
+        assert_same(gb.download(k), g1.download(0), f"instance {k} bc {p['bc']} {p['flux']}")
+        assert gb.clock(k) == g1.clock(0)
+        g1.close()
+    gb.close()
+
+
 def test_small_ensemble_matches_oracle():
     pkg, E = ens()
     params = E.ensemble_params(seed=3, n=6)
