@@ -178,17 +178,17 @@ int athmi_upload(athmi_grid* g, int inst, const double* gas, const double* b1i, 
   Clock c;
   if ((rc = g->read_clock(inst, c))) return rc;
   const Inst& I = g->hinst[inst];
-  g->stage.assign((size_t)NSTATE * I.ps, 0.0);
-  double* s = g->stage.data();
-  for (int j = 0; j < I.sy; j++)
-    for (int i = 0; i < I.sx; i++) {
-      const size_t h = (size_t)j * I.sx + i, k = (size_t)j * I.pitch + i;
-      for (int q = 0; q < 8; q++) s[q * I.ps + k] = gas[8 * h + q];
-      s[F_B1I * I.ps + k] = b1i[h];
-      s[F_B2I * I.ps + k] = b2i[h];
-      s[F_B3I * I.ps + k] = b3i[h];
-    }
-  HIP_TRY(hipMemcpy(I.state[c.cur], s, sizeof(double) * NSTATE * I.ps, hipMemcpyHostToDevice));
+  // the caller's arrays land unchanged in the idle ping-pong buffer (the next
+  // step rewrites all of it) and one kernel scatters them into the planes
+  const size_t n = (size_t)I.sy * I.sx;
+  if ((size_t)I.sy * I.pitch > I.ps) return fail(ATHMI_ESTATE, "athmi_upload: plane layout");
+  double* stage = I.state[c.cur ^ 1];
+  HIP_TRY(hipMemcpy(stage, gas, sizeof(double) * 8 * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(stage + 8 * n, b1i, sizeof(double) * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(stage + 9 * n, b2i, sizeof(double) * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(stage + 10 * n, b3i, sizeof(double) * n, hipMemcpyHostToDevice));
+  HIP_TRY(launch_state_unpack(stage, I.state[c.cur], I.ps, I.pitch, I.sx, I.sy, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
   return ATHMI_OK;
 }
 
@@ -199,17 +199,17 @@ int athmi_download(athmi_grid* g, int inst, double* gas, double* b1i, double* b2
   Clock c;
   if ((rc = g->read_clock(inst, c))) return rc;
   const Inst& I = g->hinst[inst];
-  g->stage.resize((size_t)NSTATE * I.ps);
-  double* s = g->stage.data();
-  HIP_TRY(hipMemcpy(s, I.state[c.cur], sizeof(double) * NSTATE * I.ps, hipMemcpyDeviceToHost));
-  for (int j = 0; j < I.sy; j++)
-    for (int i = 0; i < I.sx; i++) {
-      const size_t h = (size_t)j * I.sx + i, k = (size_t)j * I.pitch + i;
-      for (int q = 0; q < 8; q++) gas[8 * h + q] = s[q * I.ps + k];
-      b1i[h] = s[F_B1I * I.ps + k];
-      b2i[h] = s[F_B2I * I.ps + k];
-      b3i[h] = s[F_B3I * I.ps + k];
-    }
+  // gathered into the caller's layout in the idle ping-pong buffer, then
+  // copied out unchanged
+  const size_t n = (size_t)I.sy * I.sx;
+  if ((size_t)I.sy * I.pitch > I.ps) return fail(ATHMI_ESTATE, "athmi_download: plane layout");
+  double* stage = I.state[c.cur ^ 1];
+  HIP_TRY(launch_state_pack(I.state[c.cur], stage, I.ps, I.pitch, I.sx, I.sy, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  HIP_TRY(hipMemcpy(gas, stage, sizeof(double) * 8 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(b1i, stage + 8 * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(b2i, stage + 9 * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(b3i, stage + 10 * n, sizeof(double) * n, hipMemcpyDeviceToHost));
   return ATHMI_OK;
 }
 

@@ -36,6 +36,17 @@ inline int fail(int code, const std::string& msg) {
 
 typedef hipError_t (*LaunchFn)(const void*, int, hipStream_t);
 
+// transfer.hip: caller layout (gas[sy*sx][8] | b1i | b2i | b3i in `stage`)
+// <-> the SoA planes of one state buffer
+}  // namespace athmi_rt
+namespace athmi {
+hipError_t launch_state_unpack(const double* stage, double* st, size_t ps, int pitch, int sx,
+                               int sy, hipStream_t s);
+hipError_t launch_state_pack(const double* st, double* stage, size_t ps, int pitch, int sx,
+                             int sy, hipStream_t s);
+}  // namespace athmi
+namespace athmi_rt {
+
 // kernel sequences of the reference's routines
 const int kSeqStep[] = {SK_BEGIN_STEP, SK_PRED_X1, SK_PRED_X2, SK_CORNER_P,   SK_CORR,
                         SK_CORR2,      SK_CORNER_C, SK_FINAL,  SK_GHOST_NEXT, SK_END_STEP};
@@ -70,7 +81,6 @@ struct athmi_grid {
   bool use_graph = true;
   hipGraphExec_t graph = nullptr;
   std::vector<std::vector<double>> cpaw_soln;  // initial cpaw2d states (8 planes)
-  std::vector<double> stage;                   // host SoA staging
   void* outbuf = nullptr;                      // device staging of the output formats
   size_t outbuf_bytes = 0;
 

@@ -69,7 +69,9 @@ int athmi_set_spacing(athmi_grid *g, int inst, double dx1, double dx2);
 
 /* Host <-> device copies of one instance's full padded state (AoS <-> SoA).
  * Replace copy_to_gpu_mem / copy_to_host_mem (utils_cu.c:230, 293) without
- * their 4-cell x shift. */
+ * their 4-cell x shift. The arrays cross PCIe unchanged and the layout
+ * conversion runs on the device, staged in the instance's idle ping-pong
+ * buffer (the next step rewrites it). Both synchronise. */
 int athmi_upload(athmi_grid *g, int inst, const double *gas, const double *b1i,
                  const double *b2i, const double *b3i);
 int athmi_download(athmi_grid *g, int inst, double *gas, double *b1i, double *b2i,

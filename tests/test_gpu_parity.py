@@ -77,6 +77,33 @@ def test_exact_boundaries_match_oracle(bcs):
     gpu.close()
 
 
+def test_upload_download_round_trip_and_restep():
+    """athmi_upload / athmi_download stage the caller's arrays in the idle
+    ping-pong buffer and convert on the device: a random padded state comes
+    back bit for bit, and a state re-uploaded mid-run (its staging left in the
+    buffer the next step writes) keeps stepping identically to the oracle."""
+    pkg = load_pkg()
+    ov = ["grid/Nx1=40", "grid/Nx2=56"]
+    say("[gpu] upload/download round trip, blast 40x56")
+    gpu, par, tlim, _ = pkg.setup_problem("blast", ov, variant="exact")
+    ora, _, _ = oracle_setup("blast", ov)
+    rng = np.random.default_rng(5)
+    rand = [rng.normal(size=a.shape) for a in gpu.download(0)]
+    gpu.upload(0, *rand)
+    for name, x, y in zip(FIELDS, gpu.download(0), rand):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64)), f"round trip: {name}"
+    gpu.upload(0, *ora.get())
+    compare_exact(gpu, ora, "re-uploaded initial state")
+    gpu.set_limits(tlim, -1)
+    for s in range(6):
+        if s == 3:
+            gpu.upload(0, *gpu.download(0))
+        gpu.step()
+        ora.step(tlim)
+        compare_exact(gpu, ora, f"after {s + 1} steps")
+    gpu.close()
+
+
 def test_exact_routines_one_by_one():
     """integrate_2d, new_dt and set_bvals as separate calls (the drop-in
     granularity of the reference's main loop)."""
