@@ -69,9 +69,32 @@ athmi_grid* ensure(const athmi_ref_grid* pG) {
   return S.g;
 }
 
+static_assert(sizeof(athmi_ref_gas) == 8 * sizeof(double), "Gas is 8 packed doubles");
+
+// calloc_2d_array (ath_array.c:48-66) puts all rows of an array in one block:
+// the Grid's arrays then have the C-ABI's layout and cross without a host copy
+template <class T>
+bool one_block(T* const* rows, int sy, int sx) {
+  for (int j = 1; j < sy; j++)
+    if (rows[j] != rows[0] + (size_t)j * sx) return false;
+  return true;
+}
+
+bool grid_is_one_block(const athmi_ref_grid* pG) {
+  const int sx = pG->Nx1 + 8, sy = pG->Nx2 + 8;
+  return one_block(pG->U, sy, sx) && one_block(pG->B1i, sy, sx) &&
+         one_block(pG->B2i, sy, sx) && one_block(pG->B3i, sy, sx);
+}
+
 void to_device(const athmi_ref_grid* pG) {
   athmi_grid* g = ensure(pG);
   const int sx = pG->Nx1 + 8, sy = pG->Nx2 + 8;
+  if (grid_is_one_block(pG)) {
+    if (athmi_upload(g, 0, &pG->U[0][0].d, pG->B1i[0], pG->B2i[0], pG->B3i[0]) != ATHMI_OK)
+      die("athmi_upload");
+    if (athmi_set_clock(g, 0, pG->dt, pG->time, pG->nstep) != ATHMI_OK) die("athmi_set_clock");
+    return;
+  }
   S.gas.resize((size_t)8 * sx * sy);
   S.b1.resize((size_t)sx * sy);
   S.b2.resize((size_t)sx * sy);
@@ -93,6 +116,11 @@ void to_device(const athmi_ref_grid* pG) {
 
 void from_device(athmi_ref_grid* pG) {
   const int sx = pG->Nx1 + 8, sy = pG->Nx2 + 8;
+  if (grid_is_one_block(pG)) {
+    if (athmi_download(S.g, 0, &pG->U[0][0].d, pG->B1i[0], pG->B2i[0], pG->B3i[0]) != ATHMI_OK)
+      die("athmi_download");
+    return;
+  }
   if (athmi_download(S.g, 0, S.gas.data(), S.b1.data(), S.b2.data(), S.b3.data()) != ATHMI_OK)
     die("athmi_download");
   for (int j = 0; j < sy; j++)

@@ -1,0 +1,129 @@
+"""The reference-named drop-in (include/athena_dropin.h) driven the way the
+reference's main loop drives its CPU routines (main.c:221-318): integrate_2d,
+nstep++ and time += dt, new_dt, set_bvals(Grid*, 0), on a struct with the
+layout of Grid (athena.h:106-120) built here through ctypes. The exact variant
+must match the oracle value for value, ghosts included, and the returned dt
+exactly.
+
+Two Grid layouts: rows in one block, as calloc_2d_array makes them
+(ath_array.c:48-66; the drop-in then hands the arrays to the device as they
+are), and rows with a stride wider than a row (the drop-in's element-wise
+copy)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from helpers import INPUTS, load_pkg, oracle_setup
+
+pytestmark = pytest.mark.gpu
+
+
+class RefGas(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("d", "M1", "M2", "M3", "E", "B1c", "B2c", "B3c")]
+
+
+PD = ctypes.POINTER(ctypes.c_double)
+
+
+class RefGrid(ctypes.Structure):
+    _fields_ = [("U", ctypes.POINTER(ctypes.POINTER(RefGas))), ("B1i", ctypes.POINTER(PD)),
+                ("B2i", ctypes.POINTER(PD)), ("B3i", ctypes.POINTER(PD)),
+                ("x1_0", ctypes.c_double), ("x2_0", ctypes.c_double),
+                ("dx1", ctypes.c_double), ("dx2", ctypes.c_double),
+                ("dt", ctypes.c_double), ("time", ctypes.c_double), ("nstep", ctypes.c_int),
+                ("Nx1", ctypes.c_int), ("Nx2", ctypes.c_int), ("is_", ctypes.c_int),
+                ("ie", ctypes.c_int), ("js", ctypes.c_int), ("je", ctypes.c_int),
+                ("idisp", ctypes.c_int), ("jdisp", ctypes.c_int), ("outfilename", ctypes.c_char_p)]
+
+
+class HostGrid:
+    """Grid arrays in numpy memory; pad > 0 widens each row's storage so the
+    rows are not one block."""
+
+    def __init__(self, state, pad):
+        gas, b1, b2, b3 = state
+        sy, sx = b1.shape
+        self.sx, self.sy = sx, sy
+        self.gas = np.zeros((sy, sx + pad, 8))
+        self.b = [np.zeros((sy, sx + pad)) for _ in range(3)]
+        self.set(state)
+        self.urows = (ctypes.POINTER(RefGas) * sy)(
+            *[ctypes.cast(self.gas[j].ctypes.data, ctypes.POINTER(RefGas)) for j in range(sy)])
+        self.brows = [(PD * sy)(*[ctypes.cast(a[j].ctypes.data, PD) for j in range(sy)])
+                      for a in self.b]
+
+    def set(self, state):
+        gas, b1, b2, b3 = state
+        self.gas[:, :self.sx] = np.asarray(gas).reshape(self.sy, self.sx, 8)
+        for a, x in zip(self.b, (b1, b2, b3)):
+            a[:, :self.sx] = x
+
+    def get(self):
+        return (self.gas[:, :self.sx].copy(), *[a[:, :self.sx].copy() for a in self.b])
+
+
+def check_equal(host, ora, tag):
+    for name, x, y in zip(("gas", "B1i", "B2i", "B3i"), host.get(), ora.get()):
+        y = np.asarray(y).reshape(x.shape)
+        if not np.array_equal(x, y):
+            d = np.abs(x - y)
+            raise AssertionError(f"{tag}: {name} differs, max |diff| {d.max():.3e} at "
+                                 f"{np.unravel_index(np.argmax(d), d.shape)}")
+
+
+@pytest.mark.parametrize("pad", [0, 3])
+@pytest.mark.parametrize("problem,overrides,flux", [
+    ("blast", ["grid/Nx1=40", "grid/Nx2=56"], "roe"),
+    ("field_loop", ["grid/Nx1=48", "grid/Nx2=32"], "hlle"),
+])
+def test_dropin_main_loop_matches_oracle(problem, overrides, flux, pad):
+    pkg = load_pkg()
+    print(f"[gpu] drop-in {problem} {flux} {overrides} pad={pad}", flush=True)
+    L = pkg.lib()
+    par = pkg.read_athinput(f"{INPUTS}/athinput.{problem}", overrides)
+    d = pkg.desc_from_athinput(par, flux=flux)
+    ora, tlim, _ = oracle_setup(problem, overrides, flux=flux)
+    dt0, t0, n0 = ora.clock()
+
+    host = HostGrid(ora.get(), pad)
+    G = RefGrid()
+    G.U = host.urows
+    G.B1i, G.B2i, G.B3i = host.brows
+    G.Nx1, G.Nx2 = d["nx1"], d["nx2"]
+    G.x1_0, G.x2_0 = d["x1min"], d["x2min"]
+    G.dx1 = (d["x1max"] - d["x1min"]) / d["nx1"]
+    G.dx2 = (d["x2max"] - d["x2min"]) / d["nx2"]
+    G.is_, G.ie = 4, 4 + d["nx1"] - 1
+    G.js, G.je = 4, 4 + d["nx2"] - 1
+    G.dt, G.time, G.nstep = dt0, t0, n0
+
+    L.athmi_dropin_configure.argtypes = [ctypes.c_double, ctypes.c_double] + [ctypes.c_int] * 6
+    L.athmi_dropin_configure.restype = None
+    for name in ("integrate_2d", "new_dt", "set_bvals_init", "integrate_destruct_2d"):
+        getattr(L, name).restype = None
+    L.set_bvals.argtypes = [ctypes.POINTER(RefGrid), ctypes.c_int]
+    L.set_bvals.restype = None
+    L.integrate_init_2d.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.integrate_init_2d.restype = None
+
+    L.athmi_dropin_configure(d["gamma"], d["cour_no"], *d["bc"], pkg.FLUXES[flux],
+                             pkg.VARIANTS["exact"])
+    L.integrate_init_2d(G.Nx1, G.Nx2)
+    L.set_bvals_init(ctypes.byref(G))
+    try:
+        for s in range(6):
+            L.integrate_2d(ctypes.byref(G))
+            ora.integrate()
+            check_equal(host, ora, f"integrate_2d #{s}")
+            G.nstep += 1
+            G.time += G.dt
+            ora.set_time(G.dt, G.time, G.nstep)
+            L.new_dt(ctypes.byref(G))
+            ora.new_dt()
+            assert G.dt == ora.clock()[0], f"new_dt #{s}: {G.dt} vs {ora.clock()[0]}"
+            L.set_bvals(ctypes.byref(G), 0)
+            ora.set_bvals()
+            check_equal(host, ora, f"set_bvals #{s}")
+    finally:
+        L.integrate_destruct_2d()
