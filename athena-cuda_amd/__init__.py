@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(_HERE, "libathmi.so")
 INPUTS = os.path.join(_HERE, "inputs")
 
 BC_REFLECT, BC_OUTFLOW, BC_PERIODIC = 1, 2, 4
+FAIL_NONFINITE, FAIL_DENSITY, FAIL_PRESSURE = 1, 2, 4
 FLUX_ROE, FLUX_HLLE = 0, 1
 EXACT, FAST = 0, 1
 VARIANTS = {"exact": EXACT, "fast": FAST}
@@ -55,6 +56,7 @@ SIGNATURES = {
     "athmi_set_clock": (_I, [_P, _I, _D, _D, _I]),
     "athmi_get_clock": (_I, [_P, _I, _DP, _DP, _IP]),
     "athmi_set_limits": (_I, [_P, _I, _D, _I]),
+    "athmi_get_failure": (_I, [_P, _I, _IP, _IP]),
     "athmi_set_pause": (_I, [_P, _I, _D]),
     "athmi_set_bvals": (_I, [_P]),
     "athmi_new_dt": (_I, [_P]),
@@ -223,6 +225,15 @@ class Grid:
 
     def set_clock(self, inst, dt, time, nstep):
         _check(lib().athmi_set_clock(self._h, inst, dt, time, nstep), "set_clock")
+
+    def failure(self, inst=0):
+        """(flags, nstep): the FAIL_* bits raised by the states this instance
+        produced and the step of the first one (-1: none). A failed instance
+        no longer steps."""
+        f, n = ctypes.c_int(), ctypes.c_int()
+        _check(lib().athmi_get_failure(self._h, inst, ctypes.byref(f), ctypes.byref(n)),
+               "get_failure")
+        return f.value, n.value
 
     def set_limits(self, tlim, nlim=-1, inst=-1):
         _check(lib().athmi_set_limits(self._h, inst, tlim, nlim), "set_limits")

@@ -40,7 +40,8 @@ def _bind():
         L.athmi_adjoint_set_stream.argtypes = [_P, _P]
         L.athmi_adjoint_forward.argtypes = [_P, _P, _P]
         L.athmi_adjoint_backward.argtypes = [_P, _P, _P]
-        for n in ("create", "destroy", "set_stream", "forward", "backward"):
+        L.athmi_adjoint_generation.argtypes = [_P, ctypes.POINTER(ctypes.c_ulonglong)]
+        for n in ("create", "destroy", "set_stream", "forward", "backward", "generation"):
             getattr(L, "athmi_adjoint_" + n).restype = _I
         L._adjoint_bound = True
     return L
@@ -98,6 +99,14 @@ class Rollout:
                "athmi_adjoint_forward")
         return out
 
+    def generation(self):
+        """Completed forward passes of this handle; its checkpoints belong to
+        the latest one."""
+        gen = ctypes.c_ulonglong()
+        _check(lib().athmi_adjoint_generation(self._h, ctypes.byref(gen)),
+               "athmi_adjoint_generation")
+        return gen.value
+
     def backward(self, grad_n):
         import torch
         g = self._check_tensor(grad_n)
@@ -112,13 +121,24 @@ def _function():
     import torch
 
     class RolloutFn(torch.autograd.Function):
+        # The handle keeps the checkpoints of its latest forward pass only, so
+        # a graph whose forward pass was followed by another on the same
+        # handle cannot be reversed: its backward raises instead of reversing
+        # the other trajectory.
         @staticmethod
         def forward(ctx, state0, handle):
+            out = handle.forward(state0)
             ctx.handle = handle
-            return handle.forward(state0)
+            ctx.generation = handle.generation()
+            return out
 
         @staticmethod
         def backward(ctx, grad_n):
+            if ctx.handle.generation() != ctx.generation:
+                raise AthmiError(
+                    "rollout backward: another forward pass ran on this Rollout handle after "
+                    "the one being differentiated (the handle keeps only the latest "
+                    "checkpoints); use one handle per pending graph")
             return ctx.handle.backward(grad_n), None
 
     return RolloutFn
@@ -128,7 +148,9 @@ _FN = None
 
 
 def rollout(state0, handle):
-    """Differentiable rollout: state after handle.nsteps fixed-dt steps."""
+    """Differentiable rollout: state after handle.nsteps fixed-dt steps. The
+    backward pass must come before the next rollout on the same handle (it
+    raises AthmiError otherwise)."""
     global _FN
     if _FN is None:
         _FN = _function()

@@ -51,6 +51,7 @@ struct athmi_adjoint {
   int nbank = 1;            // banks of segment slots (2: re-run and reversal overlap)
   int cur = 0;              // host copy of the clock's current buffer
   bool have_forward = false;
+  unsigned long long generation = 0;  // completed forward passes
   double* mem = nullptr;    // checkpoints | segment states | adjoint state x2 | adjoint planes
   double* ck = nullptr;
   double* seg = nullptr;     // per bank K + 1 states: the input of each step of a segment
@@ -269,6 +270,13 @@ int athmi_adjoint_forward(athmi_adjoint* h, const double* state0, double* stateN
   if ((rc = planes_to_dense(h, stateN, live_state(h), h->g->stream))) return rc;
   HIP_TRY(hipStreamSynchronize(h->g->stream));
   h->have_forward = true;
+  h->generation++;
+  return ATHMI_OK;
+}
+
+int athmi_adjoint_generation(const athmi_adjoint* h, unsigned long long* gen) {
+  if (!h || !gen) return fail(ATHMI_EINVAL, "athmi_adjoint_generation: null argument");
+  *gen = h->generation;
   return ATHMI_OK;
 }
 

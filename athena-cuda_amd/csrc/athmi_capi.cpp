@@ -100,7 +100,11 @@ int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
     delete g;
     return fail(ATHMI_ENOMEM, "athmi_create: hipMalloc of the state failed");
   }
-  (void)hipMemset(g->dmem, 0, g->dmem_bytes);
+  if (hipMemset(g->dmem, 0, g->dmem_bytes) != hipSuccess) {
+    (void)hipFree(g->dmem);
+    delete g;
+    return fail(ATHMI_EHIP, "athmi_create: clearing the state failed");
+  }
   if (hipMalloc(&g->dclk, sizeof(Clock) * n) != hipSuccess ||
       hipMalloc(&g->dinst, sizeof(Inst) * n) != hipSuccess) {
     athmi_destroy(g);
@@ -113,13 +117,17 @@ int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
     hc[k].tlim = HUGE_VAL;
     hc[k].tpause = HUGE_VAL;
     hc[k].nlim = -1;
+    hc[k].fail_nstep = -1;
     Inst& I = g->hinst[k];
     I.state[0] = g->dmem + off[k];
     I.state[1] = I.state[0] + NSTATE * I.ps;
     I.scratch = I.state[1] + NSTATE * I.ps;
     I.clk = g->dclk + k;
   }
-  (void)hipMemcpy(g->dclk, hc.data(), sizeof(Clock) * n, hipMemcpyHostToDevice);
+  if (hipMemcpy(g->dclk, hc.data(), sizeof(Clock) * n, hipMemcpyHostToDevice) != hipSuccess) {
+    athmi_destroy(g);
+    return fail(ATHMI_EHIP, "athmi_create: writing the clocks failed");
+  }
   if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
     athmi_destroy(g);
     return fail(ATHMI_EHIP, "athmi_create: stream creation failed");
@@ -189,6 +197,12 @@ int athmi_upload(athmi_grid* g, int inst, const double* gas, const double* b1i, 
   HIP_TRY(hipMemcpy(stage + 10 * n, b3i, sizeof(double) * n, hipMemcpyHostToDevice));
   HIP_TRY(launch_state_unpack(stage, I.state[c.cur], I.ps, I.pitch, I.sx, I.sy, g->stream));
   HIP_TRY(hipStreamSynchronize(g->stream));
+  // a new state: the failure word describes states produced from it
+  if (c.fail != 0 || c.fail_nstep != -1) {
+    c.fail = 0;
+    c.fail_nstep = -1;
+    if ((rc = g->write_clock_head(inst, c))) return rc;
+  }
   return ATHMI_OK;
 }
 
@@ -232,6 +246,16 @@ int athmi_get_clock(athmi_grid* g, int inst, double* dt, double* time, int* nste
   if (dt) *dt = c.dt;
   if (time) *time = c.time;
   if (nstep) *nstep = c.nstep;
+  return ATHMI_OK;
+}
+
+int athmi_get_failure(athmi_grid* g, int inst, int* flags, int* nstep) {
+  int rc = check_inst(g, inst);
+  if (rc) return rc;
+  Clock c;
+  if ((rc = g->read_clock(inst, c))) return rc;
+  if (flags) *flags = c.fail;
+  if (nstep) *nstep = c.fail_nstep;
   return ATHMI_OK;
 }
 
@@ -320,7 +344,9 @@ int athmi_run(athmi_grid* g, int max_steps, int* steps) {
                         offsetof(Clock, steps_run), g->n, hipMemcpyDeviceToHost));
     bool running = false;
     for (const Clock& c : hc)
-      if (c.time < c.tlim && (c.nlim < 0 || c.nstep < c.nlim) && c.time < c.tpause) running = true;
+      if (c.time < c.tlim && (c.nlim < 0 || c.nstep < c.nlim) && c.time < c.tpause &&
+          c.fail == 0)
+        running = true;
     if (!running) break;
     int m = chunk;
     if (max_steps >= 0) m = std::min(m, max_steps - issued);

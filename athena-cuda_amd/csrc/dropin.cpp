@@ -1,6 +1,11 @@
 // dropin.cpp -- the reference's CPU entry points (include/athena_dropin.h)
-// over the device API. The Grid is copied to HBM, stepped there and copied
-// back, so callers keep the reference's in-place semantics.
+// over the device API. By default the Grid is copied to HBM, stepped there
+// and copied back on every call, so callers keep the reference's in-place
+// semantics. In resident mode (athmi_dropin_resident) the state stays in HBM
+// between calls, as the reference's own GPU main loop keeps it
+// (utils_cu.c:230, main.c:221-318): a call uploads only when the device does
+// not hold this Grid's current state, new_dt returns dt alone, and the host
+// arrays are refreshed by athmi_dropin_sync_host.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -25,7 +30,12 @@ struct DropinState {
   int bc[4] = {ATHMI_BC_PERIODIC, ATHMI_BC_PERIODIC, ATHMI_BC_PERIODIC, ATHMI_BC_PERIODIC};
   int flux = ATHMI_FLUX_ROE, variant = ATHMI_EXACT;
   double dx1 = 0, dx2 = 0, x1_0 = 0, x2_0 = 0;
+  double g_gamma = 0, g_cour = 0;  // EOS constants the handle was made with
   std::vector<double> gas, b1, b2, b3;
+  bool resident = false;            // keep the state in HBM between calls
+  const void* owner = nullptr;      // the Grid (its U block) the device copy belongs to
+  bool dev_valid = false;           // the device holds that Grid's current state
+  bool host_stale = false;          // the host arrays lag the device
 };
 DropinState S;
 
@@ -38,11 +48,22 @@ DropinState S;
 double eos_gamma() { return (&Gamma != nullptr) ? Gamma : S.gamma; }
 double courant() { return (&CourNo != nullptr) ? CourNo : S.cour_no; }
 
+void drop_handle() {
+  if (S.g) athmi_destroy(S.g);
+  S.g = nullptr;
+  S.dev_valid = false;
+  S.host_stale = false;
+  S.owner = nullptr;
+}
+
+// The handle for this Grid's geometry and EOS; the only place that records
+// the geometry it was made for, so a change of any of them makes a new one.
 athmi_grid* ensure(const athmi_ref_grid* pG) {
   const bool same = S.g && S.nx1 == pG->Nx1 && S.nx2 == pG->Nx2 && S.dx1 == pG->dx1 &&
-                    S.dx2 == pG->dx2 && S.x1_0 == pG->x1_0 && S.x2_0 == pG->x2_0;
+                    S.dx2 == pG->dx2 && S.x1_0 == pG->x1_0 && S.x2_0 == pG->x2_0 &&
+                    S.g_gamma == eos_gamma() && S.g_cour == courant();
   if (same) return S.g;
-  if (S.g) athmi_destroy(S.g);
+  drop_handle();
   athmi_desc d{};
   d.nx1 = pG->Nx1;
   d.nx2 = pG->Nx2;
@@ -66,6 +87,8 @@ athmi_grid* ensure(const athmi_ref_grid* pG) {
   S.dx2 = pG->dx2;
   S.x1_0 = pG->x1_0;
   S.x2_0 = pG->x2_0;
+  S.g_gamma = d.gamma;
+  S.g_cour = d.cour_no;
   return S.g;
 }
 
@@ -86,9 +109,20 @@ bool grid_is_one_block(const athmi_ref_grid* pG) {
          one_block(pG->B2i, sy, sx) && one_block(pG->B3i, sy, sx);
 }
 
+// Make the device hold pG's state and clock. The arrays cross only when the
+// device does not already hold this Grid's current state (always, outside
+// resident mode); the clock (three scalars the caller advances itself,
+// main.c:266-271) is written every time.
 void to_device(const athmi_ref_grid* pG) {
   athmi_grid* g = ensure(pG);
+  if (S.resident && S.dev_valid && S.owner == (const void*)pG->U[0]) {
+    if (athmi_set_clock(g, 0, pG->dt, pG->time, pG->nstep) != ATHMI_OK) die("athmi_set_clock");
+    return;
+  }
   const int sx = pG->Nx1 + 8, sy = pG->Nx2 + 8;
+  S.owner = (const void*)pG->U[0];
+  S.dev_valid = true;
+  S.host_stale = false;
   if (grid_is_one_block(pG)) {
     if (athmi_upload(g, 0, &pG->U[0][0].d, pG->B1i[0], pG->B2i[0], pG->B3i[0]) != ATHMI_OK)
       die("athmi_upload");
@@ -114,8 +148,9 @@ void to_device(const athmi_ref_grid* pG) {
   if (athmi_set_clock(g, 0, pG->dt, pG->time, pG->nstep) != ATHMI_OK) die("athmi_set_clock");
 }
 
-void from_device(athmi_ref_grid* pG) {
+void download_now(athmi_ref_grid* pG) {
   const int sx = pG->Nx1 + 8, sy = pG->Nx2 + 8;
+  S.host_stale = false;
   if (grid_is_one_block(pG)) {
     if (athmi_download(S.g, 0, &pG->U[0][0].d, pG->B1i[0], pG->B2i[0], pG->B3i[0]) != ATHMI_OK)
       die("athmi_download");
@@ -136,6 +171,16 @@ void from_device(athmi_ref_grid* pG) {
     }
 }
 
+// After a device call that changed the state: copy it back now, or (resident
+// mode) leave the host arrays stale until athmi_dropin_sync_host.
+void from_device(athmi_ref_grid* pG) {
+  if (S.resident) {
+    S.host_stale = true;
+    return;
+  }
+  download_now(pG);
+}
+
 }  // namespace
 
 extern "C" {
@@ -150,21 +195,29 @@ void athmi_dropin_configure(double gamma, double cour_no, int ibc_x1, int obc_x1
   S.bc[3] = obc_x2;
   S.flux = flux;
   S.variant = variant;
-  if (S.g) {
-    athmi_destroy(S.g);
-    S.g = nullptr;
-  }
+  drop_handle();
 }
 
+void athmi_dropin_resident(athmi_ref_grid* pG, int on) {
+  if (!on && S.resident && S.host_stale && pG) download_now(pG);
+  S.resident = on != 0;
+  if (!S.resident) S.dev_valid = false;
+}
+
+void athmi_dropin_sync_host(athmi_ref_grid* pG) {
+  if (pG && S.g && S.host_stale && S.owner == (const void*)pG->U[0]) download_now(pG);
+}
+
+void athmi_dropin_mark_dirty(void) { S.dev_valid = false; }
+
+// The device handle is made on the first call that passes a Grid (its
+// geometry and EOS decide it), so there is nothing to size here.
 void integrate_init_2d(int Nx1, int Nx2) {
-  S.nx1 = Nx1;
-  S.nx2 = Nx2;
+  (void)Nx1;
+  (void)Nx2;
 }
 
-void integrate_destruct_2d(void) {
-  if (S.g) athmi_destroy(S.g);
-  S.g = nullptr;
-}
+void integrate_destruct_2d(void) { drop_handle(); }
 
 void lr_states_init(int, int) {}
 void lr_states_destruct(void) {}
@@ -190,10 +243,7 @@ void set_bvals_init(athmi_ref_grid* pG) {
     S.bc[1] = par_geti((char*)"grid", (char*)"obc_x1");
     S.bc[2] = par_geti((char*)"grid", (char*)"ibc_x2");
     S.bc[3] = par_geti((char*)"grid", (char*)"obc_x2");
-    if (S.g) {
-      athmi_destroy(S.g);
-      S.g = nullptr;
-    }
+    drop_handle();
   }
 }
 

@@ -543,6 +543,23 @@ __device__ __forceinline__ double inverse_dt(const Inst& g, double d, double M1,
   return m;
 }
 
+// Health of one updated cell (FailBits): non-finite values, non-positive
+// density, or a pressure below the floor that cons->prim would apply
+// (convert_var.c:46-49 form: P = Gamma_1 (E - M^2/2d - B^2/2)).
+__device__ __forceinline__ int cell_fail_bits(const Inst& g, double d, double M1, double M2,
+                                              double M3, double E, double b1c, double b2c,
+                                              double b3c) {
+  int bits = 0;
+  if (!(isfinite(d) && isfinite(M1) && isfinite(M2) && isfinite(M3) && isfinite(E) &&
+        isfinite(b1c) && isfinite(b2c) && isfinite(b3c)))
+    bits |= FAIL_NONFINITE;
+  if (d <= 0.0) bits |= FAIL_DENSITY;
+  const double p = g.g1 * (E - 0.5 * (M1 * M1 + M2 * M2 + M3 * M3) / d -
+                           0.5 * (b1c * b1c + b2c * b2c + b3c * b3c));
+  if (p < kTiny) bits |= FAIL_PRESSURE;
+  return bits;
+}
+
 // Steps 9, 11a, 11b, 13 (:473-534) over [js,je+1]x[is,ie+1] into the other
 // ping-pong buffer, and the new_dt maximum of the updated interior. The B2c
 // update of 11a and the B1c update of 11b are dropped: Step 13 overwrites them.
@@ -602,16 +619,19 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict
         M3 = M3 - dtodx2 * (LD(F2 + 2 * ps, kp) - LD(F2 + 2 * ps, k));
         E = E - dtodx2 * (LD(F2 + 4 * ps, kp) - LD(F2 + 4 * ps, k));
         B3c = B3c - dtodx2 * (LD(F2 + 5 * ps, kp) - LD(F2 + 5 * ps, k));
+        const double B1c = 0.5 * (b1n + b1p), B2c = 0.5 * (b2n + b2p);
         ST(out + F_D * ps, k, d);
         ST(out + F_M1 * ps, k, M1);
         ST(out + F_M2 * ps, k, M2);
         ST(out + F_M3 * ps, k, M3);
         ST(out + F_E * ps, k, E);
-        ST(out + F_B1C * ps, k, 0.5 * (b1n + b1p));
-        ST(out + F_B2C * ps, k, 0.5 * (b2n + b2p));
+        ST(out + F_B1C * ps, k, B1c);
+        ST(out + F_B2C * ps, k, B2c);
         ST(out + F_B3C * ps, k, B3c);
         ST(out + F_B3I * ps, k, B3c);
         dti = inverse_dt(g, d, M1, M2, M3, E);
+        const int bits = cell_fail_bits(g, d, M1, M2, M3, E, B1c, B2c, B3c);
+        if (bits) atomicOr(&clk->fail, bits);
       }
     }
     const double m = block_max256(dti, red);
@@ -668,7 +688,8 @@ __global__ void __launch_bounds__(kGhostBlock) k_ghost(const Inst* __restrict__ 
 // ---------------------------------------------------------------------------
 // Clock kernels: one workgroup of 256 threads per instance.
 // mode 0: main-loop step (main.c:221-232): run while time < tlim and
-//         (nlim < 0 or nstep < nlim); clamp dt to reach tlim exactly. The
+//         (nlim < 0 or nstep < nlim) and no failure bit is set (FailBits);
+//         clamp dt to reach tlim exactly. The
 //         step also waits (does not run) once time >= tpause, so that the
 //         host can write the outputs due at the top of the iteration
 //         (main.c:226, output.c:372-376) before it runs.
@@ -679,7 +700,8 @@ __global__ void __launch_bounds__(256) k_begin(const Inst* __restrict__ insts, i
   if (threadIdx.x == 0) {
     int act = 1;
     if (mode == 0) {
-      act = (c->time < c->tlim) && (c->nlim < 0 || c->nstep < c->nlim) && (c->time < c->tpause);
+      act = (c->time < c->tlim) && (c->nlim < 0 || c->nstep < c->nlim) &&
+            (c->time < c->tpause) && (c->fail == 0);
       if (act && (c->tlim - c->time) < c->dt) c->dt = (c->tlim - c->time);
     }
     c->active = act;
@@ -694,6 +716,7 @@ __device__ __forceinline__ double reduce_slots(const Clock* c, double* red) {
 // mode 0 (end of a main-loop step, main.c:289-292 + new_dt.c:93-97):
 //   nstep++, time += dt, dt = MIN(2 dt, CourNo/max_dti), swap buffers
 // mode 1 (end of an unconditional integration): swap buffers only
+// Both record the step number of the first state that raised a failure bit.
 __global__ void __launch_bounds__(kDtSlots) k_end(const Inst* __restrict__ insts, int mode) {
   __shared__ double red[4];
   Clock* c = insts[blockIdx.x].clk;
@@ -706,6 +729,7 @@ __global__ void __launch_bounds__(kDtSlots) k_end(const Inst* __restrict__ insts
       c->dt = (c->nstep == 0) ? b : dmin(a, b);
       c->steps_run = c->steps_run + 1;
     }
+    if (c->fail != 0 && c->fail_nstep < 0) c->fail_nstep = mode == 0 ? c->nstep : c->nstep + 1;
     c->cur = c->cur ^ 1;
   }
 }

@@ -34,13 +34,24 @@ enum BcKind { BC_REFLECT = 1, BC_OUTFLOW = 2, BC_PERIODIC = 4 };
 
 constexpr int kDtSlots = 256;  // partial max slots of the fused dt reduction
 
+// Failure bits of an instance, raised by the step on the state it produces
+// (the reference only floors, convert_var.c:49 and new_dt.c:54-55, and steps
+// on). Once any is set the main-loop step no longer runs on that instance.
+enum FailBits {
+  FAIL_NONFINITE = 1,  // a NaN or infinity in d, M1..M3, E or the cell-centred B
+  FAIL_DENSITY = 2,    // d <= 0
+  FAIL_PRESSURE = 4,   // Gamma_1 (E - KE - ME) below TINY_NUMBER: the pressure floor engages
+};
+
 // Device-resident clock of one instance (Grid.dt/time/nstep of athena.h:113-114
 // plus the main-loop limits of main.c:217-218).
 struct Clock {
   double dt, time, tlim, cour_no;
   double tpause;  // the main-loop step does not run once time >= tpause (an output is due)
   int nstep, nlim, active, cur;  // cur: which ping-pong state buffer holds the state
-  int steps_run, pad0, pad1, pad2;
+  int fail;        // FailBits of the states produced so far (0: healthy)
+  int fail_nstep;  // nstep of the first state that raised a bit (-1: none)
+  int steps_run, pad0;
   unsigned long long divb_bits;           // max |div B| (bit pattern), diagnostics
   unsigned long long dti_slot[kDtSlots];  // bit patterns of non-negative doubles
 };

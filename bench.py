@@ -10,8 +10,10 @@ periodic, Roe flux, PLM, CFL 0.4, FP64, state resident in HBM. The exact
 bar against the reference CPU integrator.
 
 Prints one JSON line with the metric, a roofline object for the dominant kernel
-(HIP-event durations of a kernel-by-kernel pass), the step-level roofline at
-168 algorithmic bytes per cell-update, the FP64 work rate, the CPU baseline
+(the step's algorithmic bytes, 168 B per cell-update, over that kernel's
+HIP-event duration in a kernel-by-kernel pass; its own plane I/O, scratch
+included, under separate keys), the step-level roofline at 168 algorithmic
+bytes per cell-update, the FP64 work rate, the CPU baseline
 (the reference's own CPU integrator timed on this host on a bounded sample),
 the batched-ensemble rate and the adjoint/forward cost ratio.
 """
@@ -30,9 +32,11 @@ HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6    # MI355X vector FP64
 ALG_BYTES_PER_CELL = 168   # 10 FP64 read (U x8, B1i, B2i) + 11 written (U x8, B1i, B2i, B3i)
 
-# Compulsory HBM bytes per unit of work of each kernel: the planes it must read
-# once and write once (8 B each), times the units of one launch on an N1 x N2
-# grid. Stencil neighbours served from cache are not counted again.
+# Plane I/O of each kernel, scratch planes included: the planes it reads once
+# and writes once (8 B each), times the units of one launch on an N1 x N2
+# grid. Stencil neighbours served from cache are not counted again. Reported
+# as roofline.kernel_io_*; the roofline's algorithmic bytes are the metric's
+# 168 B per cell-update.
 KERNEL_IO = {
     # name: (planes read, planes written, units(N1, N2))
     "k_pred_x1": (9, 22, lambda a, b: (a + 4) * (b + 4)),       # U, B1i -> Ul,Ur,F (21) + emf3_cc
@@ -338,9 +342,14 @@ def main():
     kern = {L.athmi_step_kernel_name(k).decode(): avg[k] for k in range(16)
             if L.athmi_step_kernel_name(k)}
     dom = max(kern, key=kern.get)
-    rd, wr, units = KERNEL_IO[dom]
-    alg_bytes = (rd + wr) * 8 * units(n, n)
+    # the metric's algorithmic bytes (168 B per cell-update: the state read
+    # once and written once) over one launch of the dominant kernel: the whole
+    # step's compulsory traffic, priced against that kernel's duration alone
+    alg_bytes = ALG_BYTES_PER_CELL * n * n
     achieved = alg_bytes / (kern[dom] * 1e-3) / 1e9
+    # the kernel's own plane I/O, scratch planes included (not algorithmic)
+    rd, wr, units = KERNEL_IO[dom]
+    io_bytes = (rd + wr) * 8 * units(n, n)
     pmc = pmc_summary() if n == 4096 else None
     traffic = None
     if pmc and dom in pmc["kernels"]:
@@ -348,7 +357,13 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                 "kernel_ms": kern[dom], "alg_bytes_per_launch": alg_bytes,
-                "traffic_source": "profiles/step_kernels_blast4096_pmc.md" if traffic else None,
+                "alg_bytes_per_cell_update": ALG_BYTES_PER_CELL,
+                "kernel_io_bytes_per_launch": io_bytes,
+                "kernel_io_frac": io_bytes / (kern[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "traffic_source": None if not traffic else {
+                    "file": "profiles/step_kernels_blast4096_pmc.md",
+                    "commit": pmc.get("commit"), "workload": pmc.get("workload"),
+                    "measured_in_this_run": False},
                 "kernels_ms": kern}
     step_bw = ALG_BYTES_PER_CELL * value / 1e9
     out = {

@@ -80,6 +80,17 @@ int athmi_download(athmi_grid *g, int inst, double *gas, double *b1i, double *b2
 /* Grid.dt / Grid.time / Grid.nstep (athena.h:113-114) of one instance */
 int athmi_set_clock(athmi_grid *g, int inst, double dt, double time, int nstep);
 int athmi_get_clock(athmi_grid *g, int inst, double *dt, double *time, int *nstep);
+/* Failure word of one instance: the step checks every state it produces and
+ * raises ATHMI_FAIL_* bits; from then on the main-loop step (athmi_step,
+ * athmi_run) no longer runs on that instance, the others go on. *nstep gets
+ * the step number of the first state that raised a bit (-1: none). The
+ * reference has no such check (it floors pressure, convert_var.c:49 and
+ * new_dt.c:54-55, and steps on); athmi_upload of a new state clears the word.
+ * Either pointer may be NULL. */
+#define ATHMI_FAIL_NONFINITE 1 /* NaN or infinity in d, M1..M3, E or cell-centred B */
+#define ATHMI_FAIL_DENSITY 2   /* d <= 0 */
+#define ATHMI_FAIL_PRESSURE 4  /* Gamma_1 (E - KE - ME) < TINY_NUMBER (the floor engages) */
+int athmi_get_failure(athmi_grid *g, int inst, int *flags, int *nstep);
 /* main-loop limits (main.c:217-218: tlim, nlim; nlim < 0 = none); inst = -1: all */
 int athmi_set_limits(athmi_grid *g, int inst, double tlim, int nlim);
 /* Output cadence (main.c:226 calls data_output, output.c:362-376, at the top

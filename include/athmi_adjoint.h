@@ -35,6 +35,11 @@ int athmi_adjoint_forward(athmi_adjoint *h, const double *state0, double *stateN
 /* gradN = dL/dstateN, grad0 = dL/dstate0 (device pointers, dense layout);
  * valid after athmi_adjoint_forward with the same state0 */
 int athmi_adjoint_backward(athmi_adjoint *h, const double *gradN, double *grad0);
+/* Number of completed athmi_adjoint_forward calls on the handle. The
+ * checkpoints belong to the latest forward pass only: a caller that records
+ * the generation after its forward pass and compares it before its backward
+ * pass detects that another forward pass on the same handle came between. */
+int athmi_adjoint_generation(const athmi_adjoint *h, unsigned long long *gen);
 
 /* Per-routine vector-Jacobian products on the host (same source as the
  * adjoint kernels). Vectors use the Cons1D / Prim1D member order

@@ -134,3 +134,34 @@ def test_adjoint_gradient_vs_finite_differences_density():
     worst = _fd_check(A, h, s0, target, dirs)
     h.close()
     assert worst[1e-6] <= 1e-6
+
+
+def test_superseded_rollout_backward_raises_on_the_device_handle():
+    """Two rollouts on one handle before backward: the first graph's backward
+    raises (the handle holds the second forward's checkpoints); the second
+    graph's gradient equals a fresh handle's."""
+    import torch
+    A = adj_mod()
+    n = 16
+    s0 = rollout_state(5, n, n)
+    s1 = rollout_state(6, n, n)
+    dt = stable_dt(s0, n, n)
+    desc = dict(nx1=n, nx2=n, x1min=0.0, x1max=1.0, x2min=0.0, x2max=1.0, gamma=GAMMA,
+                bc=(4, 4, 4, 4), flux="roe")
+    print("[gpu] stale rollout backward on a device handle", flush=True)
+    dev = torch.device("cuda:0")
+    h = A.Rollout(desc, dt, 4, 2)
+    x0 = torch.from_numpy(s0).to(dev).requires_grad_(True)
+    x1 = torch.from_numpy(s1).to(dev).requires_grad_(True)
+    y0 = A.rollout(x0, h)
+    y1 = A.rollout(x1, h)
+    with pytest.raises(load_pkg().AthmiError, match="another forward pass"):
+        y0[0].sum().backward()
+    y1[0].sum().backward()
+    h2 = A.Rollout(desc, dt, 4, 2)
+    x1b = torch.from_numpy(s1).to(dev).requires_grad_(True)
+    A.rollout(x1b, h2)[0].sum().backward()
+    assert torch.equal(x1.grad, x1b.grad) or torch.allclose(x1.grad, x1b.grad, rtol=1e-13,
+                                                              atol=0.0)
+    h.close()
+    h2.close()
