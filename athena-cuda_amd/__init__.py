@@ -71,6 +71,8 @@ SIGNATURES = {
     "athmi_autotune": (_I, [_P, _I, _IP, _IP, _IP, _DP]),
     "athmi_set_tile_order": (_I, [_P, _I]),
     "athmi_set_corr_rows": (_I, [_P, _I]),
+    "athmi_set_x2_kind": (_I, [_P, _I]),
+    "athmi_get_x2_kind": (_I, [_P, _IP]),
     "athmi_get_shape": (_I, [_P, _IP, _IP, _IP, _IP]),
     "athmi_device_state": (_I, [_P, _I, ctypes.POINTER(_P), _IP]),
     "athmi_pgen_blast": (_I, [_P, _I, _D, _D, _D, _D, _D, _D, _D]),
@@ -287,6 +289,15 @@ class Grid:
         v = [ctypes.c_int() for _ in range(4)]
         _check(lib().athmi_get_shape(self._h, *[ctypes.byref(x) for x in v]), "get_shape")
         return tuple(x.value for x in v)
+
+    def set_x2_kind(self, kind):
+        """x2 predictor form: 0 = stencil in registers, 1 = stencil in LDS"""
+        _check(lib().athmi_set_x2_kind(self._h, kind), "set_x2_kind")
+
+    def x2_kind(self):
+        k = ctypes.c_int()
+        _check(lib().athmi_get_x2_kind(self._h, ctypes.byref(k)), "get_x2_kind")
+        return k.value
 
     def set_tile_order(self, xcd_remap=1):
         _check(lib().athmi_set_tile_order(self._h, int(xcd_remap)), "set_tile_order")

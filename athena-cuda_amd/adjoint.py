@@ -41,7 +41,9 @@ def _bind():
         L.athmi_adjoint_forward.argtypes = [_P, _P, _P]
         L.athmi_adjoint_backward.argtypes = [_P, _P, _P]
         L.athmi_adjoint_generation.argtypes = [_P, ctypes.POINTER(ctypes.c_ulonglong)]
-        for n in ("create", "destroy", "set_stream", "forward", "backward", "generation"):
+        L.athmi_adjoint_set_deterministic.argtypes = [_P, _I]
+        for n in ("create", "destroy", "set_stream", "forward", "backward", "generation",
+                  "set_deterministic"):
             getattr(L, "athmi_adjoint_" + n).restype = _I
         L._adjoint_bound = True
     return L
@@ -50,7 +52,7 @@ def _bind():
 class Rollout:
     """Device handle of one fixed-dt rollout with sqrt(N) checkpointing."""
 
-    def __init__(self, desc, dt, nsteps, every=None):
+    def __init__(self, desc, dt, nsteps, every=None, deterministic=False):
         L = _bind()
         every = every or max(1, int(round(math.sqrt(nsteps))))
         bc = desc.get("bc", (4, 4, 4, 4))
@@ -63,6 +65,9 @@ class Rollout:
         _check(L.athmi_adjoint_create(ctypes.byref(d), dt, nsteps, every, ctypes.byref(h)),
                "athmi_adjoint_create")
         self._h = h
+        if deterministic:
+            _check(L.athmi_adjoint_set_deterministic(h, 1), "athmi_adjoint_set_deterministic")
+        self.deterministic = bool(deterministic)
         self.desc = dict(desc)
         self.dt, self.nsteps, self.every = dt, nsteps, every
         self.shape = (11, int(desc["nx2"]) + 8, int(desc["nx1"]) + 8)

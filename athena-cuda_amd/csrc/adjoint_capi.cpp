@@ -51,6 +51,7 @@ struct athmi_adjoint {
   int nbank = 1;            // banks of segment slots (2: re-run and reversal overlap)
   int cur = 0;              // host copy of the clock's current buffer
   bool have_forward = false;
+  bool det = false;         // deterministic reverse: one stream, ordered sums
   unsigned long long generation = 0;  // completed forward passes
   double* mem = nullptr;    // checkpoints | segment states | adjoint state x2 | adjoint planes
   double* ck = nullptr;
@@ -274,6 +275,12 @@ int athmi_adjoint_forward(athmi_adjoint* h, const double* state0, double* stateN
   return ATHMI_OK;
 }
 
+int athmi_adjoint_set_deterministic(athmi_adjoint* h, int on) {
+  if (!h) return fail(ATHMI_EINVAL, "null handle");
+  h->det = on != 0;
+  return ATHMI_OK;
+}
+
 int athmi_adjoint_generation(const athmi_adjoint* h, unsigned long long* gen) {
   if (!h || !gen) return fail(ATHMI_EINVAL, "athmi_adjoint_generation: null argument");
   *gen = h->generation;
@@ -294,6 +301,13 @@ int athmi_adjoint_backward(athmi_adjoint* h, const double* gradN, double* grad0)
   a.ab = h->ab;
   a.dt = h->dt;
   a.ain = ain;
+  a.det = h->det ? 1 : 0;
+  a.color = 0;
+  // the deterministic reverse runs every kernel of a step on one stream, in
+  // the order of the two-stream schedule, with the scatters replaced by
+  // ordered gathers (fold, predictor sources) or disjoint launches (corners)
+  const int fold = h->det ? AK_FOLD_OUT_DET : AK_FOLD_OUT;
+  if (h->det) B = A;
   HIP_TRY(athmi_adjoint_launch(&a, AK_CLEAR, R));
   HIP_TRY(hipEventRecord(h->ev_sync, R));
   HIP_TRY(hipStreamWaitEvent(A, h->ev_sync, 0));
@@ -317,7 +331,7 @@ int athmi_adjoint_backward(athmi_adjoint* h, const double* gradN, double* grad0)
       a.aout = aout;
       // the previous step's x2 tail on B wrote ain planes this step folds
       HIP_TRY(hipStreamWaitEvent(A, h->ev_join2, 0));
-      for (int k : kRevHead) HIP_TRY(athmi_adjoint_launch(&a, k, A));
+      for (int k : kRevHead) HIP_TRY(athmi_adjoint_launch(&a, k == AK_FOLD_OUT ? fold : k, A));
       HIP_TRY(hipEventRecord(h->ev_fork1, A));
       HIP_TRY(hipStreamWaitEvent(B, h->ev_fork1, 0));
       HIP_TRY(athmi_adjoint_launch(&a, AK_RIEMANN_C2, B));
@@ -345,7 +359,7 @@ int athmi_adjoint_backward(athmi_adjoint* h, const double* gradN, double* grad0)
   }
   // the set_bvals that precedes the first step
   a.aout = aout;
-  HIP_TRY(athmi_adjoint_launch(&a, AK_FOLD_OUT, A));
+  HIP_TRY(athmi_adjoint_launch(&a, fold, A));
   if ((rc = planes_to_dense(h, grad0, aout, A))) return rc;
   HIP_TRY(hipStreamSynchronize(A));
   HIP_TRY(hipStreamSynchronize(R));

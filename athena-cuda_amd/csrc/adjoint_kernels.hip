@@ -80,6 +80,70 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_fold(AdjArgs a, int which) 
   }
 }
 
+// The same fold as an ordered gather (deterministic reverse): the thread of
+// non-ghost cell (j, i) sums, field by field, the adjoints of every ghost
+// whose fill copies that cell -- rows ascending, then columns -- and adds
+// them once. The set_bvals map is separable: ghost row jj maps to a row
+// independently of the column, and ghost column ii to a column, so the
+// candidates are (rows mapping to j or j itself) x (columns mapping to i or i
+// itself). Only cells within 4 of an edge can receive; the ghosts are cleared
+// afterwards by ak_fold_clear.
+__global__ void __launch_bounds__(kTileX* kTileY) ak_fold_gather(AdjArgs a) {
+  const Inst& g = a.g;
+  const int i = (int)blockIdx.x * kTileX + (int)threadIdx.x;
+  const int j = (int)blockIdx.y * kTileY + (int)threadIdx.y;
+  if (i >= g.sx || j >= g.sy) return;
+  double* A = a.aout;
+  const size_t ps = g.ps;
+  const int pitch = g.pitch;
+  for (int f = 0; f < NSTATE; f++) {
+    const int ie_x = (f == F_B1I) ? g.ie + 1 : g.ie;
+    const int je_y = (f == F_B2I) ? g.je + 1 : g.je;
+    if (i < g.is || i > ie_x || j < g.js || j > je_y) continue;  // ghosts receive nothing
+    const bool band = (j - g.js < 4) || (je_y - j < 4) || (i - g.is < 4) || (ie_x - i < 4);
+    if (!band) continue;
+    // ascending: ghost rows below, the row itself, ghost rows above
+    int rows[9], cols[9], nr = 0, nc = 0;
+    int sj, si;
+    bool neg;
+    for (int jj = 0; jj < g.js; jj++)
+      if (ghost_src(g, f, jj, g.is, sj, si, neg) && sj == j) rows[nr++] = jj;
+    rows[nr++] = j;
+    for (int jj = je_y + 1; jj < g.sy; jj++)
+      if (ghost_src(g, f, jj, g.is, sj, si, neg) && sj == j) rows[nr++] = jj;
+    for (int ii = 0; ii < g.is; ii++)
+      if (ghost_src(g, f, g.js, ii, sj, si, neg) && si == i) cols[nc++] = ii;
+    cols[nc++] = i;
+    for (int ii = ie_x + 1; ii < g.sx; ii++)
+      if (ghost_src(g, f, g.js, ii, sj, si, neg) && si == i) cols[nc++] = ii;
+    double sum = 0.0;
+    bool any = false;
+    for (int r = 0; r < nr; r++)
+      for (int c = 0; c < nc; c++) {
+        const int jj = rows[r], ii = cols[c];
+        if (jj == j && ii == i) continue;
+        if (ghost_src(g, f, jj, ii, sj, si, neg) && sj == j && si == i) {
+          const double v = A[f * ps + (uint32_t)(jj * pitch + ii)];
+          sum += neg ? -v : v;
+          any = true;
+        }
+      }
+    if (any) A[f * ps + (uint32_t)(j * pitch + i)] += sum;
+  }
+}
+__global__ void __launch_bounds__(kTileX* kTileY) ak_fold_clear(AdjArgs a) {
+  const Inst& g = a.g;
+  const int i = (int)blockIdx.x * kTileX + (int)threadIdx.x;
+  const int j = (int)blockIdx.y * kTileY + (int)threadIdx.y;
+  if (i >= g.sx || j >= g.sy) return;
+  const uint32_t k = (uint32_t)(j * g.pitch + i);
+  for (int f = 0; f < NSTATE; f++) {
+    int sj, si;
+    bool neg;
+    if (ghost_src(g, f, j, i, sj, si, neg)) a.aout[f * g.ps + k] = 0.0;
+  }
+}
+
 // ALL: every adjoint plane (start of a reverse pass); otherwise ain and the
 // NACC accumulated planes (every step)
 __global__ void __launch_bounds__(256) ak_zero(AdjArgs a, int all) {
@@ -165,6 +229,9 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_corner(AdjArgs a) {
   } else {
     if (!range_ij(g.js, g.je + 1, g.is, g.ie + 1, j, i)) return;
   }
+  // deterministic reverse: one launch per (j, i) parity class; corners of one
+  // class touch disjoint flux and emf entries, so no entry takes two adds
+  if (a.det && (((j & 1) << 1) | (i & 1)) != a.color) return;
   const size_t ps = g.ps;
   const int pitch = g.pitch;
   const uint32_t k = (uint32_t)(j * pitch + i);
@@ -550,13 +617,25 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_pred(AdjArgs a) {
   const double slb = hdt * (X1 ? wlb.by : wlb.bz), srb = hdt * (X1 ? wrb.by : wrb.bz);
   const double dblb = slb * vl / dx, vlb = slb * (b0 - bm) / dx;
   const double dbrb = srb * vr / dx, vrb = srb * (bp - b0) / dx;
-  ADD(a.ain + bplane * ps, k - stride, -dblb);
-  ADD(a.ain + bplane * ps, k, b0b + dblb - dbrb);
-  ADD(a.ain + bplane * ps, k + stride, dbrb);
-  ADD(a.ain + mplane * ps, k - stride, vlb / dl);
-  ADD(a.ain + F_D * ps, k - stride, -vlb * vl / dl);
-  ADD(a.ain + mplane * ps, k, vrb / dr);
-  ADD(a.ain + F_D * ps, k, -vrb * vr / dr);
+  if (a.det) {
+    // stored per face; ak_plm_gather adds them to the cells in a fixed order
+    double* pc = a.ab + (X1 ? A_PC1 : A_PC2) * ps;
+    pc[k] = -dblb;
+    pc[ps + k] = b0b + dblb - dbrb;
+    pc[2 * ps + k] = dbrb;
+    pc[3 * ps + k] = vlb / dl;
+    pc[4 * ps + k] = -vlb * vl / dl;
+    pc[5 * ps + k] = vrb / dr;
+    pc[6 * ps + k] = -vrb * vr / dr;
+  } else {
+    ADD(a.ain + bplane * ps, k - stride, -dblb);
+    ADD(a.ain + bplane * ps, k, b0b + dblb - dbrb);
+    ADD(a.ain + bplane * ps, k + stride, dbrb);
+    ADD(a.ain + mplane * ps, k - stride, vlb / dl);
+    ADD(a.ain + F_D * ps, k - stride, -vlb * vl / dl);
+    ADD(a.ain + mplane * ps, k, vrb / dr);
+    ADD(a.ain + F_D * ps, k, -vrb * vr / dr);
+  }
   st7(a.ab + (X1 ? A_WL1 : A_WL2) * ps, ps, k, wlb);
   st7(a.ab + (X1 ? A_WR1 : A_WR2) * ps, ps, k, wrb);
 }
@@ -642,12 +721,25 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_plm_gather(AdjArgs a) {
     any = any || wb[n] != 0.0;
   }
   double bxb = src[21 * ps + k];
-  if (!any && bxb == 0.0) return;
-  const EosC eos{g.gamma, g.g1, g.g2};
-  const double* st = a.st;
   const int q[7] = {F_D, X1 ? F_M1 : F_M2, X1 ? F_M2 : F_M3, X1 ? F_M3 : F_M1, F_E,
                     X1 ? F_B2C : F_B3C, X1 ? F_B3C : F_B1C};
   const int qx = X1 ? F_B1C : F_B2C;
+  if (a.det) {
+    // the predictor-face source adjoints of faces k + stride, k and k - stride
+    // (the deterministic reverse stores them per face instead of adding them)
+    const double* pc = a.ab + (X1 ? A_PC1 : A_PC2) * ps;
+    const int bplane = X1 ? F_B1I : F_B2I;
+    const int mplane = X1 ? F_M2 : F_M1;
+    const double bsum = pc[k + stride] + pc[ps + k] + pc[2 * ps + k - stride];
+    const double msum = pc[3 * ps + k + stride] + pc[5 * ps + k];
+    const double dsum = pc[4 * ps + k + stride] + pc[6 * ps + k];
+    if (bsum != 0.0) ADD(a.ain + bplane * ps, k, bsum);
+    if (msum != 0.0) ADD(a.ain + mplane * ps, k, msum);
+    if (dsum != 0.0) ADD(a.ain + F_D * ps, k, dsum);
+  }
+  if (!any && bxb == 0.0) return;
+  const EosC eos{g.gamma, g.g1, g.g2};
+  const double* st = a.st;
   const Cons7 u{st[q[0] * ps + k], st[q[1] * ps + k], st[q[2] * ps + k], st[q[3] * ps + k],
                 st[q[4] * ps + k], st[q[5] * ps + k], st[q[6] * ps + k]};
   Cons7 ub = zero_cons();
@@ -684,7 +776,15 @@ extern "C" hipError_t athmi_adjoint_launch(const AdjArgs* ap, int kernel, hipStr
       hipLaunchKernelGGL(ak_update, grid2(g.js - 1, g.je + 1, g.is - 1, g.ie + 1), b2, 0, s, a);
       break;
     case AK_CORNER_C:
-      hipLaunchKernelGGL(ak_corner<false>, grid2(g.js, g.je + 1, g.is, g.ie + 1), b2, 0, s, a);
+      for (int c = 0; c < (a.det ? 4 : 1); c++) {
+        AdjArgs ac = a;
+        ac.color = c;
+        hipLaunchKernelGGL(ak_corner<false>, grid2(g.js, g.je + 1, g.is, g.ie + 1), b2, 0, s, ac);
+      }
+      break;
+    case AK_FOLD_OUT_DET:
+      hipLaunchKernelGGL(ak_fold_gather, grid2(0, g.sy - 1, 0, g.sx - 1), b2, 0, s, a);
+      hipLaunchKernelGGL(ak_fold_clear, grid2(0, g.sy - 1, 0, g.sx - 1), b2, 0, s, a);
       break;
     case AK_RIEMANN_C1:
       if (g.flux == kFluxHlle)
@@ -709,7 +809,12 @@ extern "C" hipError_t athmi_adjoint_launch(const AdjArgs* ap, int kernel, hipStr
       hipLaunchKernelGGL(ak_transverse, grid2(g.js - 2, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);
       break;
     case AK_CORNER_P:
-      hipLaunchKernelGGL(ak_corner<true>, grid2(g.js - 1, g.je + 2, g.is - 1, g.ie + 2), b2, 0, s, a);
+      for (int c = 0; c < (a.det ? 4 : 1); c++) {
+        AdjArgs ac = a;
+        ac.color = c;
+        hipLaunchKernelGGL(ak_corner<true>, grid2(g.js - 1, g.je + 2, g.is - 1, g.ie + 2), b2, 0, s,
+                           ac);
+      }
       break;
     case AK_EMF_CC:
       hipLaunchKernelGGL(ak_emf_cc, grid2(g.js - 2, g.je + 2, g.is - 2, g.ie + 2), b2, 0, s, a);

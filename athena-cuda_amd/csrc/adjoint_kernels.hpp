@@ -40,7 +40,10 @@ enum AdjPlane {
   A_PW1 = 95,   // PLM reverse per cell (x1 sweep): adjoints of its 3 stencil cells'
                 // primitive states (3 x 7) and of its Bxc (1)
   A_PW2 = 117,  // ... x2 sweep
-  NADJ = 139
+  A_PC1 = 139,  // deterministic reverse: the predictor-face source adjoints of the x1
+                // sweep, per face: B(k-1), B(k), B(k+1), M(k-1), d(k-1), M(k), d(k)
+  A_PC2 = 146,  // ... x2 sweep
+  NADJ = 153
 };
 
 // Everything the adjoint kernels of one step need
@@ -54,6 +57,8 @@ struct AdjArgs {
   double* ain;         // adjoint of S^n (NSTATE planes), accumulated
   double* aout;        // adjoint of S^{n+1} (NSTATE planes)
   double dt;
+  int det;             // 1: deterministic reverse (every sum in a fixed order)
+  int color;           // corner reverse in det mode: the corner parity class run
 };
 
 enum AdjKernel {
@@ -75,6 +80,7 @@ enum AdjKernel {
   AK_PRED_X2,       // Steps 2c-2e per x2 face
   AK_PLM_X2,        // Step 2b per cell
   AK_PLMG_X2,       // Step 2a per cell
+  AK_FOLD_OUT_DET,  // AK_FOLD_OUT as an ordered gather per source cell
   AK_COUNT
 };
 

@@ -307,24 +307,41 @@ ATHMI_HD void trace_left_moving(const EigPrim& E, double evn, double hdtodx, con
 // lr_states_plm.c:95-245, loop body for one cell: from the primitive states
 // of cells i-1, i, i+1 and the cell-centred normal field, the interface states
 // Wl[i+1] (left of face i+1/2) and Wr[i] (right of face i-1/2).
-ATHMI_HD void plm_cell(const Prim7& Wm, const Prim7& W0, const Prim7& Wp, double bxc,
-                       double dtodx, const EosC& eos, Prim7& wl_next, Prim7& wr_this) {
+// The stencil (cells i-1, i, i+1) is read through `S`: S.get(Wm, W0, Wp)
+// fills the three states and S.fence() marks where earlier copies may die.
+// RegStencil holds them in registers; a kernel that keeps the stencil in LDS
+// passes a stencil that reads LDS after a fence, so the three states hold no
+// registers while the eigen-projections run and are read again for the
+// clamps. The arithmetic is the same either way.
+struct RegStencil {
+  const Prim7 &m, &z, &p;
+  ATHMI_HDM void get(Prim7& a, Prim7& b, Prim7& c) const { a = m; b = z; c = p; }
+  ATHMI_HDM void fence() const {}
+};
+template <class Stencil>
+ATHMI_HD void plm_cell_s(const Stencil& S, double bxc, double dtodx, const EosC& eos,
+                         Prim7& wl_next, Prim7& wr_this) {
   EigPrim E;
-  esys_prim(W0.d, W0.vx, W0.p, bxc, W0.by, W0.bz, eos, E);
-  const double* wm = &Wm.d;
-  const double* w0 = &W0.d;
-  const double* wp = &Wp.d;
   double dWc[7], dWl[7], dWr[7], dWg[7];
+  {
+    Prim7 Wm, W0, Wp;
+    S.get(Wm, W0, Wp);
+    esys_prim(W0.d, W0.vx, W0.p, bxc, W0.by, W0.bz, eos, E);
+    const double* wm = &Wm.d;
+    const double* w0 = &W0.d;
+    const double* wp = &Wp.d;
 #pragma unroll
-  for (int n = 0; n < 7; n++) {
-    dWc[n] = wp[n] - wm[n];
-    dWl[n] = w0[n] - wm[n];
-    dWr[n] = wp[n] - w0[n];
-    if (dWl[n] * dWr[n] > 0.0)
-      dWg[n] = 2.0 * dWl[n] * dWr[n] / (dWl[n] + dWr[n]);
-    else
-      dWg[n] = 0.0;
+    for (int n = 0; n < 7; n++) {
+      dWc[n] = wp[n] - wm[n];
+      dWl[n] = w0[n] - wm[n];
+      dWr[n] = wp[n] - w0[n];
+      if (dWl[n] * dWr[n] > 0.0)
+        dWg[n] = 2.0 * dWl[n] * dWr[n] / (dWl[n] + dWr[n]);
+      else
+        dWg[n] = 0.0;
+    }
   }
+  S.fence();
   const Vec7 dac = project_L(E, dWc);
   const Vec7 dal = project_L(E, dWl);
   const Vec7 dar = project_L(E, dWr);
@@ -340,6 +357,12 @@ ATHMI_HD void plm_cell(const Prim7& Wm, const Prim7& W0, const Prim7& Wp, double
     }
   }
   const Vec7 dWm = project_R(E, da);
+  S.fence();
+  Prim7 Wm, W0, Wp;
+  S.get(Wm, W0, Wp);
+  const double* wm = &Wm.d;
+  const double* w0 = &W0.d;
+  const double* wp = &Wp.d;
   double Wlv[7], Wrv[7], dW[7];
 #pragma unroll
   for (int n = 0; n < 7; n++) {
@@ -379,6 +402,10 @@ ATHMI_HD void plm_cell(const Prim7& Wm, const Prim7& W0, const Prim7& Wp, double
   trace_left_moving<6>(E, E.ev6, hdtodx, dW, pWl, pWr);
   wl_next = Prim7{pWl[0], pWl[1], pWl[2], pWl[3], pWl[4], pWl[5], pWl[6]};
   wr_this = Prim7{pWr[0], pWr[1], pWr[2], pWr[3], pWr[4], pWr[5], pWr[6]};
+}
+ATHMI_HD void plm_cell(const Prim7& Wm, const Prim7& W0, const Prim7& Wp, double bxc,
+                       double dtodx, const EosC& eos, Prim7& wl_next, Prim7& wr_this) {
+  plm_cell_s(RegStencil{Wm, W0, Wp}, bxc, dtodx, eos, wl_next, wr_this);
 }
 
 // ---------------------------------------------------------------------------

@@ -31,6 +31,24 @@ namespace ATHMI_VARIANT {
 
 __device__ __forceinline__ int dcdiv(int a, int b) { return (a + b - 1) / b; }
 
+// A value the same in every lane, moved to scalar registers (the compiler
+// cannot always prove that values read through the instance descriptor are
+// uniform, and keeps them in vector registers otherwise)
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ double uni(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffff));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <class T>
+__device__ __forceinline__ T* uni(T* p) {
+  const unsigned long long b = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(b & 0xffffffffu));
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return (T*)(((unsigned long long)hi << 32) | lo);
+}
+
 // The hardware deals workgroups to the 8 XCDs round-robin (workgroup b to XCD
 // b % 8). With remap, tile t of n is renumbered so that each XCD gets one
 // contiguous run of tiles: row neighbours and the rows above and below then
@@ -82,6 +100,13 @@ __device__ __forceinline__ void lds_put64(double (*s)[kX2Block], int t, const Pr
 // Keeps the compiler from forwarding earlier loads into a recomputation
 // (emits no instruction)
 __device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
+// A phase boundary: memory accesses stay on their side (compiler_fence) and
+// the scheduler moves no instruction across, so the live values of two
+// phases are never interleaved (emits no instruction)
+__device__ __forceinline__ void phase_fence() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_sched_barrier(0);
+}
 
 __device__ __forceinline__ void put7(double* base, size_t ps, uint32_t k, const Cons7& u) {
   ST(base + 0 * ps, k, u.d); ST(base + 1 * ps, k, u.mx); ST(base + 2 * ps, k, u.my);
@@ -132,6 +157,17 @@ struct LdsPark {
   __device__ void fence() const { compiler_fence(); }
 };
 
+// The x1 PLM's stencil (cells tid-1, tid, tid+1 of the row in LDS), read
+// again after each fence so that it holds no registers in between
+struct RowStencil {
+  double (*s)[kX1Block];
+  int t;
+  __device__ void get(Prim7& m, Prim7& z, Prim7& p) const {
+    m = lds_prim(s, t - 1); z = lds_prim(s, t); p = lds_prim(s, t + 1);
+  }
+  __device__ void fence() const { compiler_fence(); }
+};
+
 // SAVE: also store the primitive face states (Inst::prims). FLUX: the Riemann
 // solver; a launch takes only the tiles of instances that use it, and with one
 // solver per kernel the predictor fits two waves per SIMD.
@@ -147,22 +183,25 @@ __global__ void __launch_bounds__(kX1Block, 2) k_pred_x1(const Inst* __restrict_
     const Inst& g = insts[find_inst(insts, ninst, K_PRED_X1, t)];
     const Clock* clk = g.clk;
     if (!clk->active || g.flux != FLUX) continue;
-    const int local = t - g.tile0[K_PRED_X1];
-    const int nseg = dcdiv(g.nx1 + 4, kX1Faces);
+    const int local = uni(t - g.tile0[K_PRED_X1]);
+    const int gis = uni(g.is), gie = uni(g.ie);
+    const int nseg = dcdiv(uni(g.nx1) + 4, kX1Faces);
     const int row = local / nseg, seg = local - row * nseg;
-    const int j = g.js - 2 + row;
-    const int f0 = g.is - 1 + seg * kX1Faces;
+    const int j = uni(g.js) - 2 + row;
+    const int f0 = gis - 1 + seg * kX1Faces;
     const int cell = f0 - 2 + tid;
-    const double dt = clk->dt;
-    const double dtodx1 = dt / g.dx1;
+    const double dt = uni(clk->dt);
+    const double dx1 = uni(g.dx1);
+    const double dtodx1 = dt / dx1;
     const double hdt = 0.5 * dt;
-    const EosC eos{g.gamma, g.g1, g.g2};
-    const double* __restrict__ st = g.state[clk->cur];
-    double* __restrict__ sc = g.scratch;
-    const size_t ps = g.ps;
-    const uint32_t rk = (uint32_t)(j * g.pitch);
+    const EosC eos{uni(g.gamma), uni(g.g1), uni(g.g2)};
+    const double* __restrict__ st = uni(g.state[clk->cur]);
+    double* __restrict__ sc = uni(g.scratch);
+    double* __restrict__ pw = uni(g.prims);
+    const size_t ps = (size_t)uni((int)g.ps);
+    const uint32_t rk = (uint32_t)(j * uni(g.pitch));
     double bxc = 0.0;
-    if (cell >= g.is - 3 && cell <= g.ie + 3) {
+    if (cell >= gis - 3 && cell <= gie + 3) {
       const uint32_t k = rk + (uint32_t)cell;
       const double d = LD(st + F_D * ps, k), m1 = LD(st + F_M1 * ps, k), m2 = LD(st + F_M2 * ps, k);
       const double b1c = LD(st + F_B1C * ps, k), b2c = LD(st + F_B2C * ps, k);
@@ -170,19 +209,18 @@ __global__ void __launch_bounds__(kX1Block, 2) k_pred_x1(const Inst* __restrict_
       bxc = b1c;
       lds_put(sw, tid, cons_to_prim(u, bxc, eos));
       // Step 3 (:249-255): emf3 at t^n over [jl,ju] x [il,iu]
-      if (cell >= g.is - 2 && cell <= g.ie + 2 && tid >= 1 && tid <= kX1Block - 2)
+      if (cell >= gis - 2 && cell <= gie + 2 && tid >= 1 && tid <= kX1Block - 2)
         ST(sc + S_ECC * ps, k, (b1c * m2 - b2c * m1) / d);
     }
     __syncthreads();
-    if (tid >= 1 && tid <= kX1Block - 2 && cell >= g.is - 2 && cell <= g.ie + 2) {
+    if (tid >= 1 && tid <= kX1Block - 2 && cell >= gis - 2 && cell <= gie + 2) {
       Prim7 wl, wr;
-      plm_cell(lds_prim(sw, tid - 1), lds_prim(sw, tid), lds_prim(sw, tid + 1), bxc, dtodx1, eos,
-               wl, wr);
+      plm_cell_s(RowStencil{sw, tid}, bxc, dtodx1, eos, wl, wr);
       lds_put(swl, tid + 1, wl);
       lds_put(swr, tid, wr);
     }
     __syncthreads();
-    if (tid >= 2 && tid <= kX1Block - 2 && cell <= g.ie + 2) {
+    if (tid >= 2 && tid <= kX1Block - 2 && cell <= gie + 2) {
       const uint32_t k = rk + (uint32_t)cell;
       // face states: PLM output + MHD source terms for 1/2 dt (:156-164) +
       // prim->cons (:175-176); rebuilt from LDS and memory for the HLLE fallback
@@ -192,9 +230,9 @@ __global__ void __launch_bounds__(kX1Block, 2) k_pred_x1(const Inst* __restrict_
         wr = lds_prim(swr, tid);
         const double bm = LD(st + F_B1I * ps, k - 1), bp = LD(st + F_B1I * ps, k + 1);
         b0 = LD(st + F_B1I * ps, k);
-        double src = (LD(st + F_M2 * ps, k - 1) / LD(st + F_D * ps, k - 1)) * (b0 - bm) / g.dx1;
+        double src = (LD(st + F_M2 * ps, k - 1) / LD(st + F_D * ps, k - 1)) * (b0 - bm) / dx1;
         wl.by = wl.by + hdt * src;
-        src = (LD(st + F_M2 * ps, k) / LD(st + F_D * ps, k)) * (bp - b0) / g.dx1;
+        src = (LD(st + F_M2 * ps, k) / LD(st + F_D * ps, k)) * (bp - b0) / dx1;
         wr.by = wr.by + hdt * src;
         ul = prim_to_cons(wl, b0, eos);
         ur = prim_to_cons(wr, b0, eos);
@@ -204,8 +242,8 @@ __global__ void __launch_bounds__(kX1Block, 2) k_pred_x1(const Inst* __restrict_
       double b0;
       build(ul, ur, wl, wr, b0);
       if (SAVE) {
-        put7(g.prims + P_WL1 * ps, ps, k, wl);
-        put7(g.prims + P_WR1 * ps, ps, k, wr);
+        put7(pw + P_WL1 * ps, ps, k, wl);
+        put7(pw + P_WR1 * ps, ps, k, wr);
       }
       put7(sc + S_UL1 * ps, ps, k, ul);
       put7(sc + S_UR1 * ps, ps, k, ur);
@@ -317,6 +355,132 @@ __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ i
 }
 
 // ---------------------------------------------------------------------------
+// Steps 2a-2e again, for two waves per SIMD: the same march down a segment of
+// x2 faces, but nothing of the stencil stays in registers across the Riemann
+// flux. Each lane keeps five 7-double slots in LDS: the primitive states of
+// rows j-1, j (and j+1 while the PLM of cell j runs), the pending left state
+// Wl(j) and the right state Wr(j). Per face: convert row j+1 into the free
+// slot, PLM of cell j (its Wl(j+1) overwrites the dead row j-1), then the face
+// state build and the flux reading Wl(j), Wr(j) back from LDS. The slots
+// rotate: row j-1's slot becomes the next pending Wl, Wl(j)'s the next free.
+// LDS: 5 x 7 x 64 doubles = 17.5 KiB per wave (eight waves per CU).
+constexpr int kX2Slots = 5;  // 4 rotating + the Wr slot
+template <bool SAVE, int FLUX>
+__global__ void __launch_bounds__(kX2Block, 2) k_pred_x2l(const Inst* __restrict__ insts,
+                                                          int ninst, int ntiles, int seg_len,
+                                                          int remap) {
+  __shared__ double slots[kX2Slots][7][kX2Block];
+  const int lane = threadIdx.x;
+  auto put = [&](int s, const Prim7& w) {
+    slots[s][0][lane] = w.d; slots[s][1][lane] = w.vx; slots[s][2][lane] = w.vy;
+    slots[s][3][lane] = w.vz; slots[s][4][lane] = w.p; slots[s][5][lane] = w.by;
+    slots[s][6][lane] = w.bz;
+  };
+  auto get = [&](int s) -> Prim7 {
+    return Prim7{slots[s][0][lane], slots[s][1][lane], slots[s][2][lane], slots[s][3][lane],
+                 slots[s][4][lane], slots[s][5][lane], slots[s][6][lane]};
+  };
+  constexpr int kWr = 4;
+  // the PLM's stencil, read from the slots again after each fence
+  typedef decltype(get) Getter;
+  struct LdsStencil {
+    const Getter* rd;
+    int a, b, c;
+    __device__ void get(Prim7& m, Prim7& z, Prim7& p) const {
+      m = (*rd)(a); z = (*rd)(b); p = (*rd)(c);
+    }
+    __device__ void fence() const { compiler_fence(); }
+  };
+  for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
+    const int t = xcd_tile(t0, ntiles, remap);
+    const Inst& g = insts[find_inst(insts, ninst, K_PRED_X2, t)];
+    const Clock* clk = g.clk;
+    if (!clk->active || g.flux != FLUX) continue;
+    const int local = uni(t - g.tile0[K_PRED_X2]);
+    const int ncb = dcdiv(uni(g.nx1) + 5, kX2Block);
+    const int sg = local / ncb, cb = local - sg * ncb;
+    const int i = uni(g.is) - 2 + cb * kX2Block + lane;
+    if (i > uni(g.ie) + 2) continue;
+    const int jf0 = uni(g.js) - 1 + sg * seg_len;
+    const int jf1 = min(jf0 + seg_len - 1, uni(g.je) + 2);
+    const double dt = uni(clk->dt);
+    const double dx2 = uni(g.dx2);
+    const double dtodx2 = dt / dx2;
+    const double hdt = 0.5 * dt;
+    const EosC eos{uni(g.gamma), uni(g.g1), uni(g.g2)};
+    const double* __restrict__ st = uni(g.state[clk->cur]);
+    double* __restrict__ sc = uni(g.scratch);
+    double* __restrict__ pw = uni(g.prims);
+    const size_t ps = (size_t)uni((int)g.ps);
+    const int pitch = uni(g.pitch);
+    // cons->prim of row jj in the permuted order (d, M2, M3, M1, E, B3c, B1c), Bx = B2c
+    auto prim_at = [&](int jj) -> Prim7 {
+      const uint32_t k = (uint32_t)(jj * pitch + i);
+      const Cons7 u{LD(st + F_D * ps, k), LD(st + F_M2 * ps, k), LD(st + F_M3 * ps, k), LD(st + F_M1 * ps, k),
+                    LD(st + F_E * ps, k), LD(st + F_B3C * ps, k), LD(st + F_B1C * ps, k)};
+      return cons_to_prim(u, LD(st + F_B2C * ps, k), eos);
+    };
+    auto bx_at = [&](int jj) { return LD(st + F_B2C * ps, (uint32_t)(jj * pitch + i)); };
+    // start of the march: rows jf0-2, jf0-1, jf0 and the PLM of cell jf0-1
+    put(0, prim_at(jf0 - 2));
+    put(1, prim_at(jf0 - 1));
+    put(2, prim_at(jf0));
+    {
+      Prim7 wl, wr_unused;
+      plm_cell_s(LdsStencil{&get, 0, 1, 2}, bx_at(jf0 - 1), dtodx2, eos, wl, wr_unused);
+      compiler_fence();
+      put(0, wl);
+    }
+    int sP = 1, sQ = 2, sW = 0, sF = 3;  // rows j-1, j; pending Wl(j); free
+    for (int j = jf0; j <= jf1; j++) {
+      phase_fence();
+      put(sF, prim_at(j + 1));
+      phase_fence();
+      {
+        Prim7 wl_next, wr_cur;
+        const LdsStencil sten{&get, sP, sQ, sF};
+        plm_cell_s(sten, bx_at(j), dtodx2, eos, wl_next, wr_cur);
+        compiler_fence();
+        put(kWr, wr_cur);
+        put(sP, wl_next);
+      }
+      phase_fence();
+      const uint32_t k = (uint32_t)(j * pitch + i);
+      const int sw = sW;
+      // face states: PLM output + MHD source terms for 1/2 dt (:216-224) +
+      // prim->cons (:235-236); rebuilt from LDS and memory for the HLLE fallback
+      auto build = [&](Cons7& ul, Cons7& ur, Prim7& wl, Prim7& wr, double& b0) {
+        compiler_fence();
+        wl = get(sw);
+        wr = get(kWr);
+        const double bm = LD(st + F_B2I * ps, k - pitch), bp = LD(st + F_B2I * ps, k + pitch);
+        b0 = LD(st + F_B2I * ps, k);
+        double src = (LD(st + F_M1 * ps, k - pitch) / LD(st + F_D * ps, k - pitch)) * (b0 - bm) / dx2;
+        wl.bz = wl.bz + hdt * src;
+        src = (LD(st + F_M1 * ps, k) / LD(st + F_D * ps, k)) * (bp - b0) / dx2;
+        wr.bz = wr.bz + hdt * src;
+        ul = prim_to_cons(wl, b0, eos);
+        ur = prim_to_cons(wr, b0, eos);
+      };
+      Cons7 ul, ur;
+      Prim7 wl, wr;
+      double b0;
+      build(ul, ur, wl, wr, b0);
+      if (SAVE) {
+        put7(pw + P_WL2 * ps, ps, k, wl);
+        put7(pw + P_WR2 * ps, ps, k, wr);
+      }
+      put7(sc + S_UL2 * ps, ps, k, ul);
+      put7(sc + S_UR2 * ps, ps, k, ur);
+      put7(sc + S_F2 * ps, ps, k, riemann_of<FLUX>(ul, ur, wl, wr, b0, eos, build));
+      // rotate: row j -> j-1, row j+1 -> j, Wl(j+1) pending, Wl(j)'s slot free
+      const int nP = sQ, nQ = sF, nW = sP, nF = sW;
+      sP = nP; sQ = nQ; sW = nW; sF = nF;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // integrate_emf3_corner (__integrate_2d__.c:632-702): upwinded corner emf from
 // the face fluxes (x1-flux of B2 is -E3, x2-flux of B1 is +E3) and the
 // cell-centred emf3. PRED: Step 4 over [js-1,je+2]x[is-1,ie+2] from predictor
@@ -391,20 +555,21 @@ __global__ void __launch_bounds__(kTileX* TY, 2) k_corr(const Inst* __restrict__
     const Inst& g = insts[find_inst(insts, ninst, K_CORR, t)];
     const Clock* clk = g.clk;
     if (!clk->active || g.flux != FLUX) continue;
-    const int local = t - g.tile0[K_CORR];
-    const int ntx = dcdiv(g.nx1 + 3, kTileX);
+    const int local = uni(t - g.tile0[K_CORR]);
+    const int gis = uni(g.is), gjs = uni(g.js);
+    const int ntx = dcdiv(uni(g.nx1) + 3, kTileX);
     const int ty = local / ntx, tx = local - ty * ntx;
-    const int i = g.is - 1 + tx * kTileX + threadIdx.x;
-    const int j = g.js - 1 + ty * TY + threadIdx.y;
-    if (i > g.ie + 1 || j > g.je + 1) continue;
-    const double dt = clk->dt;
-    const double dtodx1 = dt / g.dx1, dtodx2 = dt / g.dx2;
+    const int i = gis - 1 + tx * kTileX + threadIdx.x;
+    const int j = gjs - 1 + ty * TY + threadIdx.y;
+    if (i > uni(g.ie) + 1 || j > uni(g.je) + 1) continue;
+    const double dt = uni(clk->dt);
+    const double dtodx1 = dt / uni(g.dx1), dtodx2 = dt / uni(g.dx2);
     const double hdtodx1 = 0.5 * dtodx1, hdtodx2 = 0.5 * dtodx2;
-    const EosC eos{g.gamma, g.g1, g.g2};
-    const double* __restrict__ st = g.state[clk->cur];
-    double* __restrict__ sc = g.scratch;
-    const size_t ps = g.ps;
-    const int pitch = g.pitch;
+    const EosC eos{uni(g.gamma), uni(g.g1), uni(g.g2)};
+    const double* __restrict__ st = uni(g.state[clk->cur]);
+    double* __restrict__ sc = uni(g.scratch);
+    const size_t ps = (size_t)uni((int)g.ps);
+    const int pitch = uni(g.pitch);
     const uint32_t k = (uint32_t)(j * pitch + i);
     const double* emf = sc + S_EMF * ps;
     const double* B1i = st + F_B1I * ps;
@@ -413,7 +578,7 @@ __global__ void __launch_bounds__(kTileX* TY, 2) k_corr(const Inst* __restrict__
     auto b2h = [&](uint32_t kk) { return LD(B2i, kk) + hdtodx1 * (LD(emf, kk + 1) - LD(emf, kk)); };
     const double* F1 = sc + S_F1 * ps;
     const double* F2 = sc + S_F2 * ps;
-    if (PART == 1 && i >= g.is) {
+    if (PART == 1 && i >= gis) {
       // corrected x1-face states (Steps 5a, 5b) at the half-step face field;
       // rebuilt from memory for the HLLE fallback
       auto build = [&](Cons7& L, Cons7& R, Prim7& wl, Prim7& wr, double& bx) {
@@ -465,7 +630,7 @@ __global__ void __launch_bounds__(kTileX* TY, 2) k_corr(const Inst* __restrict__
       put7(sc + S_F1C * ps, ps, k, riemann_of<FLUX>(L, R, wl, wr, bx, eos, build, park));
     }
     if (PART == 1) continue;
-    if (j >= g.js) {
+    if (j >= gjs) {
       // corrected x2-face states (Steps 6a, 6b) at the half-step face field
       auto build = [&](Cons7& L, Cons7& R, Prim7& wl, Prim7& wr, double& bx) {
         compiler_fence();
@@ -813,7 +978,12 @@ static PredX1Kernel pred_x1_kernel(bool save, int flux) {
   if (flux == kFluxHlle) return save ? k_pred_x1<true, kFluxHlle> : k_pred_x1<false, kFluxHlle>;
   return save ? k_pred_x1<true, kFluxRoe> : k_pred_x1<false, kFluxRoe>;
 }
-static PredX2Kernel pred_x2_kernel(bool save, int flux) {
+static PredX2Kernel pred_x2_kernel(bool save, int flux, int kind) {
+  if (kind == 1) {
+    if (flux == kFluxHlle)
+      return save ? k_pred_x2l<true, kFluxHlle> : k_pred_x2l<false, kFluxHlle>;
+    return save ? k_pred_x2l<true, kFluxRoe> : k_pred_x2l<false, kFluxRoe>;
+  }
   if (flux == kFluxHlle) return save ? k_pred_x2<true, kFluxHlle> : k_pred_x2<false, kFluxHlle>;
   return save ? k_pred_x2<true, kFluxRoe> : k_pred_x2<false, kFluxRoe>;
 }
@@ -875,8 +1045,10 @@ hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
     case SK_PRED_X2_SAVE:
       for (int f = 0; f < 2; f++) {
         if (!(b.flux_mask & (1 << f))) continue;
-        const PredX2Kernel fn = pred_x2_kernel(kernel == SK_PRED_X2_SAVE, f);
-        hipLaunchKernelGGL(fn, dim3(grid_of(b, K_PRED_X2, 16)), dim3(kX2Block), 0, s, b.insts,
+        const int kind = b.shape.x2_kind;
+        const PredX2Kernel fn = pred_x2_kernel(kernel == SK_PRED_X2_SAVE, f, kind);
+        hipLaunchKernelGGL(fn, dim3(grid_of(b, K_PRED_X2, kind == 1 ? 8 : 16)), dim3(kX2Block),
+                           0, s, b.insts,
                            b.ninst, b.ntiles[K_PRED_X2], b.shape.x2_seg, b.shape.xcd_remap);
       }
       break;

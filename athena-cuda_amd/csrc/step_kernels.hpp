@@ -118,6 +118,8 @@ struct LaunchShape {
   int persist;       // 0: one tile per workgroup; >0: workgroups per CU of a persistent grid
   int xcd_remap;     // 1: contiguous runs of tiles per XCD (L2 locality), 0: launch order
   int corr_rows;     // rows of a corrector tile (kTileX x corr_rows faces): 1, 2 or 4
+  int x2_kind;       // x2 predictor: 0 = stencil marched in registers (one wave per
+                     // SIMD), 1 = stencil marched in LDS slots (two waves per SIMD)
 };
 
 // Everything a launcher needs for one batch (device descriptor array, tile

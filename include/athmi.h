@@ -130,12 +130,19 @@ int athmi_set_tile_order(athmi_grid *g, int xcd_remap);
 /* Rows of a corrector tile (64 x rows faces: 1, 2 or 4; default 4). Results
  * are identical. */
 int athmi_set_corr_rows(athmi_grid *g, int rows);
+/* Form of the x2 predictor: 0 = each lane marches its column with the PLM
+ * stencil in registers (one wave per SIMD), 1 = the stencil and pending face
+ * states in LDS slots (two waves per SIMD). Picked by the autotuner; results
+ * are identical. */
+int athmi_set_x2_kind(athmi_grid *g, int kind);
+int athmi_get_x2_kind(const athmi_grid *g, int *kind);
 /* The current launch shape (any pointer may be NULL) */
 int athmi_get_shape(const athmi_grid *g, int *x2_seg, int *persist, int *xcd_remap,
                     int *corr_rows);
 int athmi_use_graph(athmi_grid *g, int on);
-/* Autotuner: time every launch shape (x2 marching depth 16..128 x persistent
- * workgroups per CU 0..32 x tile order, then the corrector's tile height
+/* Autotuner: time every launch shape (x2 predictor form x x2 marching depth
+ * 16..128 x persistent workgroups per CU 0..32 x tile order, then the
+ * corrector's tile height
  * 4/2/1 at the best of those) over `steps` main-loop steps from the current
  * state through the step graph, restore the state and clocks; the four
  * fastest shapes then run three more trials each, in turn, and the lowest

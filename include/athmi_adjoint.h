@@ -35,6 +35,14 @@ int athmi_adjoint_forward(athmi_adjoint *h, const double *state0, double *stateN
 /* gradN = dL/dstateN, grad0 = dL/dstate0 (device pointers, dense layout);
  * valid after athmi_adjoint_forward with the same state0 */
 int athmi_adjoint_backward(athmi_adjoint *h, const double *gradN, double *grad0);
+/* on = 1: the reverse pass sums every adjoint in a fixed order, so two
+ * backward passes of one forward pass give bitwise equal gradients. It runs
+ * on one stream, folds the ghost adjoints and the predictor-face source
+ * terms as ordered gathers and reverses the corner emf in four disjoint
+ * launches; on = 0 (the default) uses two streams and FP64 atomic adds,
+ * whose order varies at roundoff level. No reference counterpart (the
+ * reference has no derivatives). */
+int athmi_adjoint_set_deterministic(athmi_adjoint *h, int on);
 /* Number of completed athmi_adjoint_forward calls on the handle. The
  * checkpoints belong to the latest forward pass only: a caller that records
  * the generation after its forward pass and compares it before its backward

@@ -15,6 +15,13 @@
 
 using namespace ora;
 
+namespace ora {
+BranchLog& branch_log() {
+  static BranchLog L;
+  return L;
+}
+}  // namespace ora
+
 namespace {
 template <int N>
 void pack(Dual* out, const double* v, const double* t) {
@@ -130,6 +137,32 @@ void ora_tlm_rollout(int nx1, int nx2, double x1min, double x1max, double x2min,
       dout[q * np + k] = p[k].t;
     }
   }
+}
+
+// The same rollout with the branch log on: *digest and *count fingerprint
+// every branch decision the rollout made on primal values (limiter and clamp
+// selections, tracing signs, eigensystem degeneracies, upwinding and the HLLE
+// fallback of flux_roe, corner-emf upwinding, |x|). Two states whose rollouts
+// take the same branches give the same digest. The OpenMP thread count must
+// be the same for the runs compared.
+void ora_tlm_rollout_branches(int nx1, int nx2, double x1min, double x1max, double x2min,
+                              double x2max, double gamma, const int* bc, int flux, double dt,
+                              int nsteps, const double* state, const double* dstate,
+                              double* out, double* dout, unsigned long long* digest,
+                              unsigned long long* count) {
+  BranchLog& L = branch_log();
+  L.reset();
+  L.on = true;
+  ora_tlm_rollout(nx1, nx2, x1min, x1max, x2min, x2max, gamma, bc, flux, dt, nsteps, state,
+                  dstate, out, dout);
+  L.on = false;
+  unsigned long long h = 0, n = 0;
+  for (const BranchLog::Slot& sl : L.slot) {
+    h += sl.h;
+    n += sl.n;
+  }
+  *digest = h;
+  *count = n;
 }
 
 // FP64 operations of nsteps calls of integrate_2d on a state (11 planes, as

@@ -4,7 +4,7 @@ this container only) run at the stated sizes and lengths of the workloads:
 
   blast 400x600 on [-0.5,0.5]x[-0.75,0.75] to t = 0.2     (Roe, HLLE)
   field_loop 128x64 on [-1,1]x[-0.5,0.5] to t = 2          (Roe, HLLE)
-  cpaw2d N x N, N = 64 .. 512, to t = 1 (one wave period)  (Roe, HLLE)
+  cpaw2d N x N, N = 64 .. 1024, to t = 1 (one wave period) (Roe, HLLE)
 
 The full states are too large to commit, and the GPU's exact variant equals
 the reference value for value, so each checkpoint stores the SHA-256 of the
@@ -38,6 +38,9 @@ CASES = [("blast_400x600", "blast", f, [], 100) for f in ("roe", "hlle")]
 CASES += [("field_loop_128x64", "field_loop", f, [], 100) for f in ("roe", "hlle")]
 CASES += [(f"cpaw2d_{n}", "cpaw2d", f, [f"grid/Nx1={n}", f"grid/Nx2={n}"], 100)
           for n in (64, 128, 256, 512) for f in ("roe", "hlle")]
+# one period at 1024^2 (about 2000 steps; half an hour per flux on one core)
+CASES += [("cpaw2d_1024", "cpaw2d", f, ["grid/Nx1=1024", "grid/Nx2=1024"], 1024)
+          for f in ("roe", "hlle")]
 
 
 def state_digest(gas, b1i, b2i, b3i):
@@ -103,8 +106,11 @@ def main():
         print(f"{res['name']} {res['flux']}: {res['steps']} steps", flush=True)
         return res
 
+    # the longest cases first, so that they overlap the short ones
+    order = sorted(range(len(CASES)), key=lambda k: "1024" not in CASES[k][0])
     with ThreadPoolExecutor(jobs) as ex:
-        cases = list(ex.map(run, range(len(CASES))))
+        done = dict(zip(order, ex.map(run, order)))
+    cases = [done[k] for k in range(len(CASES))]
     meta = {"source": "reference CPU sources compiled by oracle/Makefile (oracle/_ref), "
                       "main loop of src/main.c:221-318 (ONLY_HOST)",
             "digest": "sha256 over Gas[sy][sx][8], B1i, B2i, B3i [sy][sx] as <f8, -0.0 -> +0.0",

@@ -67,6 +67,8 @@ def oracle_lib():
         L.ora_tlm_flux.restype = _I
         L.ora_tlm_lr_states.argtypes = [_D, _I, _P, _P, _P, _P, _D, _I, _I, _P, _P, _P, _P]
         L.ora_tlm_rollout.argtypes = [_I, _I, _D, _D, _D, _D, _D, _P, _I, _D, _I, _P, _P, _P, _P]
+        L.ora_tlm_rollout_branches.argtypes = [_I, _I, _D, _D, _D, _D, _D, _P, _I, _D, _I, _P, _P,
+                                               _P, _P, _P, _P]
         L.ora_pgen_blast_at.argtypes = [_P, _D, _D, _D, _D, _D, _D, _D]
         L.ora_pgen_field_loop.argtypes = [_P, _D, _D, _D, _I]
         L.ora_pgen_cpaw2d.argtypes = [_P, _D, _D, _D, _D, _D]

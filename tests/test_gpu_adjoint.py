@@ -90,28 +90,72 @@ def _fd_check(A, h, s0, target, dirs, eps_list=(1e-6,)):
     return worst
 
 
-def test_adjoint_gradient_vs_finite_differences_smooth():
-    """A generic smooth state at 64^2, 32 steps with a checkpoint every 8;
-    loss = 0.5 ||rho(T) - rho_target||^2; central differences along seeded
-    random directions in all fields. The PLM limiter makes the map piecewise
-    smooth: near extrema of the characteristic differences a perturbation of
-    size eps switches branches (dal*dar > 0, the min selections) in a number
-    of cells proportional to eps, so the FD error falls linearly with eps
-    while the adjoint is the exact derivative of the branch taken (pinned by
-    the TLM dot-product tests above)."""
+def branch_digest(O, s, n, dt, nsteps, flux="roe", bc=(4, 4, 4, 4)):
+    """Fingerprint of every branch the oracle's rollout from s takes (limiter
+    and clamp selections, tracing signs, eigensystem degeneracies, Roe
+    upwinding and HLLE fallback, corner-emf upwinding, |x|); the oracle's
+    forward is the GPU forward bit for bit."""
+    out, dout, z = np.zeros_like(s), np.zeros_like(s), np.zeros_like(s)
+    dig, cnt = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+    O.ora_tlm_rollout_branches(n, n, 0.0, 1.0, 0.0, 1.0, GAMMA, (ctypes.c_int * 4)(*bc),
+                               0 if flux == "roe" else 1, dt, nsteps, p(s), p(z), p(out),
+                               p(dout), ctypes.byref(dig), ctypes.byref(cnt))
+    return dig.value, cnt.value
+
+
+def test_adjoint_gradient_vs_finite_differences_where_no_branch_flips():
+    """A generic smooth state at 32^2, 16 steps (checkpoint every 4), loss =
+    0.5 ||rho(T) - rho_target||^2, 32 seeded directions in all fields, central
+    differences at eps = 1e-6, 1e-7, 1e-8. The step is piecewise smooth (the
+    PLM limiter, clamps, upwinding, |lambda|, the HLLE fallback); for each
+    direction and eps the oracle's branch fingerprints at x + eps v and
+    x - eps v tell whether the difference quotient straddles a kink. Where
+    nothing flips, FD must agree with the adjoint to 1e-6 relative (the
+    north-star bar); directions that flip are reported."""
+    import torch
     A = adj_mod()
-    n, nsteps, every = 64, 32, 8
+    O = oracle_lib()
+    n, nsteps, every = 32, 16, 4
     s0 = rollout_state(21, n, n)
     dt = stable_dt(s0, n, n)
     h = A.Rollout(dict(nx1=n, nx2=n, x1min=0.0, x1max=1.0, x2min=0.0, x2max=1.0, gamma=GAMMA,
                        bc=(4, 4, 4, 4), flux="roe"), dt, nsteps, every)
     target = rollout_state(22, n, n)[0][4:n + 4, 4:n + 4]
+    dev = torch.device("cuda:0")
+    tgt = torch.from_numpy(target).to(dev)
+    inner = (slice(4, n + 4), slice(4, n + 4))
+
+    def loss_of(state):
+        return 0.5 * ((h.forward(state)[0][inner] - tgt) ** 2).sum().item()
+
+    x = torch.from_numpy(s0).to(dev).requires_grad_(True)
+    (0.5 * ((A.rollout(x, h)[0][inner] - tgt) ** 2).sum()).backward()
+    grad = x.grad.detach()
     rng = np.random.default_rng(3)
-    worst = _fd_check(A, h, s0, target, [rng.normal(size=s0.shape) for _ in range(4)],
-                      eps_list=(1e-6, 1e-8))
+    flat, flipped = [], {}
+    for k in range(32):
+        v_np = rng.normal(size=s0.shape)
+        v = torch.from_numpy(v_np).to(dev)
+        ad = (grad * v).sum().item()
+        for eps in (1e-6, 1e-7, 1e-8):
+            fd = (loss_of(x.detach() + eps * v) - loss_of(x.detach() - eps * v)) / (2 * eps)
+            rel = abs(fd - ad) / abs(ad)
+            flip = (branch_digest(O, s0 + eps * v_np, n, dt, nsteps) !=
+                    branch_digest(O, s0 - eps * v_np, n, dt, nsteps))
+            if flip:
+                flipped.setdefault(eps, []).append(rel)
+            else:
+                flat.append((k, eps, rel))
+    for eps in (1e-6, 1e-7, 1e-8):
+        f = flipped.get(eps, [])
+        ok = [r for _, e, r in flat if e == eps]
+        print(f"[gpu] eps {eps:.0e}: {len(ok)} directions without a branch flip (max rel "
+              f"{max(ok) if ok else float('nan'):.2e}), {len(f)} with flips (max rel "
+              f"{max(f) if f else 0.0:.2e})", flush=True)
     h.close()
-    assert worst[1e-8] <= 1e-4
-    assert worst[1e-8] <= worst[1e-6] / 10
+    assert len(flat) >= 32, "too few flip-free directions for the check to mean anything"
+    worst = max(flat, key=lambda t: t[2])
+    assert worst[2] <= 1e-6, f"direction {worst[0]} eps {worst[1]}: rel {worst[2]:.3e}"
 
 
 def test_adjoint_gradient_vs_finite_differences_density():
@@ -161,7 +205,45 @@ def test_superseded_rollout_backward_raises_on_the_device_handle():
     h2 = A.Rollout(desc, dt, 4, 2)
     x1b = torch.from_numpy(s1).to(dev).requires_grad_(True)
     A.rollout(x1b, h2)[0].sum().backward()
-    assert torch.equal(x1.grad, x1b.grad) or torch.allclose(x1.grad, x1b.grad, rtol=1e-13,
-                                                              atol=0.0)
+    # the reverse scatters with atomic adds: two handles agree to roundoff of
+    # the gradient's scale, not bit for bit
+    err = (x1.grad - x1b.grad).abs().max().item()
+    assert err <= 1e-12 * x1b.grad.abs().max().item(), err
     h.close()
     h2.close()
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[2]], ids=lambda c: c[0])
+def test_deterministic_reverse_is_bitwise_repeatable(case):
+    """With the deterministic reverse two backward passes of one forward pass
+    give bitwise equal gradients; they agree with the default (atomic-add)
+    reverse to roundoff and satisfy the dot-product identity against the TLM."""
+    import torch
+    name, n1, n2, flux, bc, nsteps, every, blast = case
+    print(f"[gpu] deterministic reverse {name} {n1}x{n2} N={nsteps}", flush=True)
+    A = adj_mod()
+    s0 = rollout_state(hash(name) % 1000, n1, n2, blast=blast)
+    dt = stable_dt(s0, n1, n2)
+    desc = dict(nx1=n1, nx2=n2, x1min=0.0, x1max=1.0, x2min=0.0, x2max=1.0, gamma=GAMMA,
+                bc=bc, flux=flux)
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(2)
+    yb = rng.normal(size=s0.shape)
+    xd = rng.normal(size=s0.shape)
+    hd = A.Rollout(desc, dt, nsteps, every, deterministic=True)
+    hd.forward(torch.from_numpy(s0).to(dev))
+    g1 = hd.backward(torch.from_numpy(yb).to(dev)).cpu().numpy()
+    g2 = hd.backward(torch.from_numpy(yb).to(dev)).cpu().numpy()
+    assert np.array_equal(g1, g2), "deterministic backward passes differ"
+    h = A.Rollout(desc, dt, nsteps, every)
+    h.forward(torch.from_numpy(s0).to(dev))
+    g0 = h.backward(torch.from_numpy(yb).to(dev)).cpu().numpy()
+    assert np.abs(g1 - g0).max() <= 1e-12 * np.abs(g0).max()
+    O = oracle_lib()
+    out, dout = np.zeros_like(s0), np.zeros_like(s0)
+    O.ora_tlm_rollout(n1, n2, 0.0, 1.0, 0.0, 1.0, GAMMA, (ctypes.c_int * 4)(*bc),
+                      0 if flux == "roe" else 1, dt, nsteps, p(s0), p(xd), p(out), p(dout))
+    lhs, rhs = yb.ravel() @ dout.ravel(), g1.ravel() @ xd.ravel()
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    hd.close()
+    h.close()

@@ -41,12 +41,14 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("x2_kind", [0, 1])
 @pytest.mark.parametrize("flux", ["roe", "hlle"])
 @pytest.mark.parametrize("problem,overrides,nsteps", CASES)
-def test_exact_steps_match_oracle(problem, overrides, nsteps, flux):
+def test_exact_steps_match_oracle(problem, overrides, nsteps, flux, x2_kind):
     pkg = load_pkg()
-    say(f"[gpu] exact {problem} {flux} {overrides} steps={nsteps}")
+    say(f"[gpu] exact {problem} {flux} {overrides} steps={nsteps} x2 predictor form {x2_kind}")
     gpu, par, tlim, nlim = pkg.setup_problem(problem, overrides, flux=flux, variant="exact")
+    gpu.set_x2_kind(x2_kind)
     ora, tlim_o, _ = oracle_setup(problem, overrides, flux=flux)
     compare_exact(gpu, ora, "initial state")
     gpu.set_limits(tlim, nsteps)
