@@ -80,6 +80,8 @@ SIGNATURES = {
     "athmi_pgen_cpaw2d": (_I, [_P, _I, _D, _D, _D, _D, _D, _DP]),
     "athmi_cpaw2d_l1": (_I, [_P, _I, _DP]),
     "athmi_pgen_device": (_I, [_P, _I, _I, _P, _DP]),
+    "athmi_integrate_static": (_I, [ctypes.POINTER(_Desc), _D, _P, _P, _I, ctypes.c_size_t,
+                                    _P]),
     "athmi_last_error": (ctypes.c_char_p, []),
 }
 

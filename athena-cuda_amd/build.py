@@ -47,7 +47,8 @@ def build(verbose_resources=False):
             run(cmd)
             objs.append(o)
         for src in ("athmi_capi.cpp", "calib.hip", "adjoint_host.cpp", "adjoint_capi.cpp",
-                    "adjoint_kernels.hip", "output.hip", "pgen.hip", "transfer.hip"):
+                    "adjoint_kernels.hip", "output.hip", "pgen.hip", "transfer.hip",
+                    "static_step.hip"):
             o = os.path.join(tmp, src.split(".")[0] + ".o")
             # adjoint kernels: the PLM reverse's loops over the seven
             # characteristic fields are fully unrolled (each eigenmatrix entry

@@ -198,6 +198,25 @@ int athmi_pgen_device(athmi_grid *g, int first, int n, const athmi_pgen_params *
  * Gas order (d, M1, M2, M3, E, B1c, B2c, B3c), each divided by Nx1*Nx2 */
 int athmi_cpaw2d_l1(athmi_grid *g, int inst, double *out);
 
+/* integrate_2d of one grid on caller-owned device planes, allocating nothing.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc). state_in and state_out each hold the 11 planes (d, M1,
+ * M2, M3, E, B1c, B2c, B3c, B1i, B2i, B3i) of [Nx2+8][pitch] doubles,
+ * plane_stride doubles apart, ghost zones included; state_in is read,
+ * state_out receives the state athmi_integrate would produce with this dt
+ * (interior updated, ghost zones carried unchanged). The work is queued on
+ * `hip_stream` (NULL: the null stream) and not synchronised. The descriptor,
+ * clock and the 59 scratch planes live in one static device array of
+ * ATHMI_STATIC_SCRATCH_MB MiB per GPU, held until the process ends; a grid
+ * needs 4 KiB + 59 * plane_stride * 8 bytes of it (plane_stride up to about
+ * 284k doubles: e.g. 400x600 with pitch 416) and larger ones are rejected
+ * with ATHMI_EINVAL. Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (other streams, other threads) race. */
+#define ATHMI_STATIC_SCRATCH_MB 128
+int athmi_integrate_static(const athmi_desc *desc, double dt, const double *state_in,
+                           double *state_out, int pitch, size_t plane_stride, void *hip_stream);
+
 const char *athmi_last_error(void);
 
 #ifdef __cplusplus
