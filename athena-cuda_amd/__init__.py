@@ -73,6 +73,8 @@ SIGNATURES = {
     "athmi_set_corr_rows": (_I, [_P, _I]),
     "athmi_set_x2_kind": (_I, [_P, _I]),
     "athmi_get_x2_kind": (_I, [_P, _IP]),
+    "athmi_set_fusion": (_I, [_P, _I]),
+    "athmi_get_fusion": (_I, [_P, _IP]),
     "athmi_get_shape": (_I, [_P, _IP, _IP, _IP, _IP]),
     "athmi_device_state": (_I, [_P, _I, ctypes.POINTER(_P), _IP]),
     "athmi_pgen_blast": (_I, [_P, _I, _D, _D, _D, _D, _D, _D, _D]),
@@ -299,6 +301,15 @@ class Grid:
     def x2_kind(self):
         k = ctypes.c_int()
         _check(lib().athmi_get_x2_kind(self._h, ctypes.byref(k)), "get_x2_kind")
+        return k.value
+
+    def set_fusion(self, depth):
+        """0: corner emf of Step 9 in its own kernel, 1: inside the final kernel"""
+        _check(lib().athmi_set_fusion(self._h, depth), "set_fusion")
+
+    def fusion(self):
+        k = ctypes.c_int()
+        _check(lib().athmi_get_fusion(self._h, ctypes.byref(k)), "get_fusion")
         return k.value
 
     def set_tile_order(self, xcd_remap=1):

@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_fold(AdjArgs a, int which) 
 // them once. The set_bvals map is separable: ghost row jj maps to a row
 // independently of the column, and ghost column ii to a column, so the
 // candidates are (rows mapping to j or j itself) x (columns mapping to i or i
-// itself). Only cells within 4 of an edge can receive; the ghosts are cleared
+// itself). Only cells near an edge can receive; the ghosts are cleared
 // afterwards by ak_fold_clear.
 __global__ void __launch_bounds__(kTileX* kTileY) ak_fold_gather(AdjArgs a) {
   const Inst& g = a.g;
@@ -100,7 +100,9 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_fold_gather(AdjArgs a) {
     const int ie_x = (f == F_B1I) ? g.ie + 1 : g.ie;
     const int je_y = (f == F_B2I) ? g.je + 1 : g.je;
     if (i < g.is || i > ie_x || j < g.js || j > je_y) continue;  // ghosts receive nothing
-    const bool band = (j - g.js < 4) || (je_y - j < 4) || (i - g.is < 4) || (ie_x - i < 4);
+    // receivers lie within 4 cells of an edge of the cell-centred range
+    // [js, je] x [is, ie] (face planes: one more cell on the far side)
+    const bool band = (j - g.js < 5) || (je_y - j < 5) || (i - g.is < 5) || (ie_x - i < 5);
     if (!band) continue;
     // ascending: ghost rows below, the row itself, ghost rows above
     int rows[9], cols[9], nr = 0, nc = 0;

@@ -66,6 +66,7 @@ int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
   g->bl.shape.xcd_remap = 1;
   g->bl.shape.corr_rows = kTileY;
   g->bl.shape.x2_kind = 1;  // the two-wave x2 predictor (1.79 against 2.05 ms at 4096^2)
+  g->bl.shape.fuse_corner = 0;
   // device memory: per instance 2*NSTATE state planes + NSCRATCH scratch planes
   std::vector<size_t> off(n);
   size_t total = 0;
@@ -523,6 +524,14 @@ int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd
     sh.corr_rows = rows;
     rc = sweep(sh);
   }
+  // fusion depth: the Step 9 corner emf in its own kernel or inside the
+  // final kernel's tiles
+  const LaunchShape base2 = best_shape;
+  if (!rc) {
+    LaunchShape sh = base2;
+    sh.fuse_corner = base2.fuse_corner ? 0 : 1;
+    rc = sweep(sh);
+  }
   // single short trials differ by about 1% from run to run: the fastest few
   // shapes run again, in turn, and the lowest total time wins
   if (!rc && timed.size() > 1) {
@@ -567,6 +576,20 @@ int athmi_set_x2_kind(athmi_grid* g, int kind) {
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->bl.shape.x2_kind = kind;
   return g->refresh_tiles();
+}
+
+int athmi_set_fusion(athmi_grid* g, int depth) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  if (depth != 0 && depth != 1) return fail(ATHMI_EINVAL, "athmi_set_fusion: depth must be 0 or 1");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->bl.shape.fuse_corner = depth;
+  return g->refresh_tiles();
+}
+
+int athmi_get_fusion(const athmi_grid* g, int* depth) {
+  if (!g || !depth) return fail(ATHMI_EINVAL, "athmi_get_fusion: null argument");
+  *depth = g->bl.shape.fuse_corner;
+  return ATHMI_OK;
 }
 
 int athmi_get_x2_kind(const athmi_grid* g, int* kind) {

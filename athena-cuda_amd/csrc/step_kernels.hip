@@ -481,6 +481,37 @@ __global__ void __launch_bounds__(kX2Block, 2) k_pred_x2l(const Inst* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// integrate_emf3_corner (__integrate_2d__.c:632-702) at corner (j, i): the
+// x1-flux of B2 is -E3 and the x2-flux of B1 is +E3; each of the four edge
+// estimates is upwinded by the sign of the face mass flux (>0, <0, ==0).
+// F1, F2: Cons1D flux groups (d = 0, By = 5, Bz = 6); ecc: cell-centred emf3.
+__device__ __forceinline__ double corner_emf(const double* F1, const double* F2,
+                                             const double* ecc, size_t ps, int pitch, int j,
+                                             int i) {
+  const uint32_t k = (uint32_t)(j * pitch + i);
+  const uint32_t kjm = k - pitch, kim = k - 1, kjmim = k - pitch - 1;
+  const double f1d_jm = LD(F1, kjm), f1d = LD(F1, k);
+  const double f1by_jm = LD(F1 + 5 * ps, kjm), f1by = LD(F1 + 5 * ps, k);
+  const double f2d_im = LD(F2, kim), f2d = LD(F2, k);
+  const double f2bz_im = LD(F2 + 6 * ps, kim), f2bz = LD(F2 + 6 * ps, k);
+  const double e_jmim = LD(ecc, kjmim), e_jm = LD(ecc, kjm), e_im = LD(ecc, kim), e_00 = LD(ecc, k);
+  double emf_l1, emf_r1, emf_l2, emf_r2;
+  if (f1d_jm > 0.0) emf_l2 = -f1by_jm + (f2bz_im - e_jmim);
+  else if (f1d_jm < 0.0) emf_l2 = -f1by_jm + (f2bz - e_jm);
+  else emf_l2 = -f1by_jm + 0.5 * (f2bz_im - e_jmim + f2bz - e_jm);
+  if (f1d > 0.0) emf_r2 = -f1by + (f2bz_im - e_im);
+  else if (f1d < 0.0) emf_r2 = -f1by + (f2bz - e_00);
+  else emf_r2 = -f1by + 0.5 * (f2bz_im - e_im + f2bz - e_00);
+  if (f2d_im > 0.0) emf_l1 = f2bz_im + (-f1by_jm - e_jmim);
+  else if (f2d_im < 0.0) emf_l1 = f2bz_im + (-f1by - e_im);
+  else emf_l1 = f2bz_im + 0.5 * (-f1by_jm - e_jmim - f1by - e_im);
+  if (f2d > 0.0) emf_r1 = f2bz + (-f1by_jm - e_jm);
+  else if (f2d < 0.0) emf_r1 = f2bz + (-f1by - e_00);
+  else emf_r1 = f2bz + 0.5 * (-f1by_jm - e_jm - f1by - e_00);
+  return 0.25 * (emf_l1 + emf_r1 + emf_l2 + emf_r2);
+}
+
+// ---------------------------------------------------------------------------
 // integrate_emf3_corner (__integrate_2d__.c:632-702): upwinded corner emf from
 // the face fluxes (x1-flux of B2 is -E3, x2-flux of B1 is +E3) and the
 // cell-centred emf3. PRED: Step 4 over [js-1,je+2]x[is-1,ie+2] from predictor
@@ -506,28 +537,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_corner(const Inst* __restric
     const double* F1 = sc + (PRED ? S_F1 : S_F1C) * ps;
     const double* F2 = sc + (PRED ? S_F2 : S_F2C) * ps;
     const double* ecc = sc + (PRED ? S_ECC : S_ECH) * ps;
-    const uint32_t k = (uint32_t)(j * pitch + i);
-    const uint32_t kjm = k - pitch, kim = k - 1, kjmim = k - pitch - 1;
-    // component offsets within a Cons1D group: d = 0, By = 5, Bz = 6
-    const double f1d_jm = LD(F1, kjm), f1d = LD(F1, k);
-    const double f1by_jm = LD(F1 + 5 * ps, kjm), f1by = LD(F1 + 5 * ps, k);
-    const double f2d_im = LD(F2, kim), f2d = LD(F2, k);
-    const double f2bz_im = LD(F2 + 6 * ps, kim), f2bz = LD(F2 + 6 * ps, k);
-    const double e_jmim = LD(ecc, kjmim), e_jm = LD(ecc, kjm), e_im = LD(ecc, kim), e_00 = LD(ecc, k);
-    double emf_l1, emf_r1, emf_l2, emf_r2;
-    if (f1d_jm > 0.0) emf_l2 = -f1by_jm + (f2bz_im - e_jmim);
-    else if (f1d_jm < 0.0) emf_l2 = -f1by_jm + (f2bz - e_jm);
-    else emf_l2 = -f1by_jm + 0.5 * (f2bz_im - e_jmim + f2bz - e_jm);
-    if (f1d > 0.0) emf_r2 = -f1by + (f2bz_im - e_im);
-    else if (f1d < 0.0) emf_r2 = -f1by + (f2bz - e_00);
-    else emf_r2 = -f1by + 0.5 * (f2bz_im - e_im + f2bz - e_00);
-    if (f2d_im > 0.0) emf_l1 = f2bz_im + (-f1by_jm - e_jmim);
-    else if (f2d_im < 0.0) emf_l1 = f2bz_im + (-f1by - e_im);
-    else emf_l1 = f2bz_im + 0.5 * (-f1by_jm - e_jmim - f1by - e_im);
-    if (f2d > 0.0) emf_r1 = f2bz + (-f1by_jm - e_jm);
-    else if (f2d < 0.0) emf_r1 = f2bz + (-f1by - e_00);
-    else emf_r1 = f2bz + 0.5 * (-f1by_jm - e_jm - f1by - e_00);
-    ST(sc + S_EMF * ps, k, 0.25 * (emf_l1 + emf_r1 + emf_l2 + emf_r2));
+    ST(sc + S_EMF * ps, (uint32_t)(j * pitch + i), corner_emf(F1, F2, ecc, ps, pitch, j, i));
   }
 }
 
@@ -728,9 +738,16 @@ __device__ __forceinline__ int cell_fail_bits(const Inst& g, double d, double M1
 // Steps 9, 11a, 11b, 13 (:473-534) over [js,je+1]x[is,ie+1] into the other
 // ping-pong buffer, and the new_dt maximum of the updated interior. The B2c
 // update of 11a and the B1c update of 11b are dropped: Step 13 overwrites them.
+// FUSED: the tile first computes the Step 9 corner emf it reads -- (kTileY+1)
+// x (kTileX+1) corners -- into LDS from the corrector fluxes, instead of
+// reading the plane k_corner<false> wrote (one launch and the corner plane's
+// write and read fewer).
+constexpr int kCornerRows = kTileY + 1, kCornerCols = kTileX + 1;
+template <bool FUSED>
 __global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict__ insts,
                                                            int ninst, int ntiles, int remap) {
   __shared__ double red[4];
+  __shared__ double cemf[FUSED ? kCornerRows * kCornerCols : 1];
   for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
     const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, K_FINAL, t)];
@@ -739,8 +756,24 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict
     const int local = t - g.tile0[K_FINAL];
     const int ntx = dcdiv(g.nx1 + 1, kTileX);
     const int ty = local / ntx, tx = local - ty * ntx;
-    const int i = g.is + tx * kTileX + threadIdx.x;
-    const int j = g.js + ty * kTileY + threadIdx.y;
+    const int i0 = g.is + tx * kTileX, j0 = g.js + ty * kTileY;
+    const int i = i0 + threadIdx.x;
+    const int j = j0 + threadIdx.y;
+    if (FUSED) {
+      // corners [js,je+1] x [is,ie+1] of this tile's rows j0..j0+kTileY and
+      // columns i0..i0+kTileX (outside that range: never read)
+      const double* sc = g.scratch;
+      const size_t ps = g.ps;
+      const int tid = threadIdx.y * kTileX + threadIdx.x;
+      for (int c = tid; c < kCornerRows * kCornerCols; c += kTileX * kTileY) {
+        const int cj = j0 + c / kCornerCols, ci = i0 + c % kCornerCols;
+        double e = 0.0;
+        if (cj <= g.je + 1 && ci <= g.ie + 1)
+          e = corner_emf(sc + S_F1C * ps, sc + S_F2C * ps, sc + S_ECH * ps, ps, g.pitch, cj, ci);
+        cemf[c] = e;
+      }
+      __syncthreads();
+    }
     double dti = 0.0;
     if (i <= g.ie + 1 && j <= g.je + 1) {
       const double dt = clk->dt;
@@ -753,7 +786,11 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict
       const int pitch = g.pitch;
       const uint32_t k = (uint32_t)(j * pitch + i);
       const double* emf = sc + S_EMF * ps;
-      const double e00 = LD(emf, k), e10 = LD(emf, k + pitch), e01 = LD(emf, k + 1);
+      const int cc = threadIdx.y * kCornerCols + threadIdx.x;
+      auto corner = [&](int dj, int di) {
+        return FUSED ? cemf[cc + dj * kCornerCols + di] : LD(emf, k + dj * pitch + di);
+      };
+      const double e00 = corner(0, 0), e10 = corner(1, 0), e01 = corner(0, 1);
       double b1n = 0.0, b2n = 0.0;
       if (j <= g.je) {
         b1n = LD(st + F_B1I * ps, k) - dtodx2 * (e10 - e00);
@@ -764,7 +801,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict
         ST(out + F_B2I * ps, k, b2n);
       }
       if (j <= g.je && i <= g.ie) {
-        const double e11 = LD(emf, k + pitch + 1);
+        const double e11 = corner(1, 1);
         const double b1p = LD(st + F_B1I * ps, k + 1) - dtodx2 * (e11 - e01);
         const double b2p = LD(st + F_B2I * ps, k + pitch) + dtodx1 * (e11 - e10);
         const double* F1 = sc + S_F1C * ps;
@@ -1072,11 +1109,14 @@ hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
       break;
     }
     case SK_CORNER_C:
+      // with the fused final kernel the corners are computed there
+      if (b.shape.fuse_corner) break;
       hipLaunchKernelGGL(k_corner<false>, dim3(grid_of(b, K_CORNER_C, 8)), b2, 0, s, b.insts,
                          b.ninst, b.ntiles[K_CORNER_C], b.shape.xcd_remap);
       break;
     case SK_FINAL:
-      hipLaunchKernelGGL(k_final, dim3(grid_of(b, K_FINAL, 8)), b2, 0, s, b.insts, b.ninst,
+      hipLaunchKernelGGL(b.shape.fuse_corner ? k_final<true> : k_final<false>,
+                         dim3(grid_of(b, K_FINAL, 8)), b2, 0, s, b.insts, b.ninst,
                          b.ntiles[K_FINAL], b.shape.xcd_remap);
       break;
     case SK_GHOST_CUR:

@@ -120,6 +120,8 @@ struct LaunchShape {
   int corr_rows;     // rows of a corrector tile (kTileX x corr_rows faces): 1, 2 or 4
   int x2_kind;       // x2 predictor: 0 = stencil marched in registers (one wave per
                      // SIMD), 1 = stencil marched in LDS slots (two waves per SIMD)
+  int fuse_corner;   // 1: the Step 9 corner emf is computed inside the final kernel's
+                     // tiles (no k_corner<corr> launch, no corner plane round trip)
 };
 
 // Everything a launcher needs for one batch (device descriptor array, tile

@@ -134,9 +134,10 @@ def batched_leg(pkg, args, plain_rate):
     shape = None
     if not args.no_tune:
         shape = gb.autotune(2)
-        shape = gb.launch_shape() + (shape[3],)
+        shape = gb.launch_shape() + (shape[3], gb.x2_kind(), gb.fusion())
         log(f"[bench] batch shape x2_seg={shape[0]} persist={shape[1]} xcd_remap={shape[2]} "
-            f"corr_rows={shape[3]} ({shape[4]:.3f} ms/step)")
+            f"corr_rows={shape[3]} x2_form={shape[5]} fusion={shape[6]} "
+            f"({shape[4]:.3f} ms/step)")
     for _ in range(args.batch_warmup):
         gb.step()
     gb.sync()
@@ -159,7 +160,8 @@ def batched_leg(pkg, args, plain_rate):
                         "off-centre blasts, Roe even / HLLE odd, per-instance CFL dt, "
                         "one HIP graph per step",
             "shape": None if shape is None else {"x2_seg": shape[0], "persist": shape[1],
-                                                 "xcd_remap": shape[2], "corr_rows": shape[3]}}
+                                                 "xcd_remap": shape[2], "corr_rows": shape[3],
+                                                 "x2_form": shape[5], "fusion": shape[6]}}
 
 
 def adjoint_leg(pkg, args):
@@ -279,6 +281,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--batch-warmup", type=int, default=3)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--shape", default=None,
+                    help="x2_seg,persist,xcd_remap,corr_rows,x2_form,fusion: use this launch "
+                         "shape for the plain step instead of autotuning (profiling runs)")
     ap.add_argument("--adjoint", type=int, default=1, help="0: skip the adjoint leg")
     ap.add_argument("--adj-size", type=int, default=512)
     ap.add_argument("--adj-steps", type=int, default=256)
@@ -316,11 +321,21 @@ def main():
                                         variant=args.variant)
     g.set_limits(1e30, -1)  # fixed number of steps: the loop never reaches tlim
     shape = None
-    if not args.no_tune:
+    if args.shape:
+        v = [int(x) for x in args.shape.split(",")]
+        g.set_shape(v[0], v[1])
+        g.set_tile_order(v[2])
+        g.set_corr_rows(v[3])
+        g.set_x2_kind(v[4])
+        g.set_fusion(v[5])
+        shape = tuple(v[:4]) + (float("nan"), v[4], v[5])
+        log(f"[bench] fixed shape {args.shape}")
+    elif not args.no_tune:
         shape = g.autotune(3)
-        shape = g.launch_shape() + (shape[3],)
+        shape = g.launch_shape() + (shape[3], g.x2_kind(), g.fusion())
         log(f"[bench] autotuned shape x2_seg={shape[0]} persist={shape[1]} xcd_remap={shape[2]} "
-            f"corr_rows={shape[3]} ({shape[4]:.3f} ms/step)")
+            f"corr_rows={shape[3]} x2_form={shape[5]} fusion={shape[6]} "
+            f"({shape[4]:.3f} ms/step)")
     for _ in range(args.warmup):
         g.step()
     g.sync()
@@ -375,7 +390,8 @@ def main():
                                "CTU+CT, periodic, CFL 0.4; one step = integrate + new_dt + "
                                "set_bvals", "nx1": n, "nx2": n, "variant": args.variant,
                    "shape": None if shape is None else {"x2_seg": shape[0], "persist": shape[1],
-                                                 "xcd_remap": shape[2], "corr_rows": shape[3]}},
+                                                 "xcd_remap": shape[2], "corr_rows": shape[3],
+                                                 "x2_form": shape[5], "fusion": shape[6]}},
         "roofline": roofline,
         "step_roofline": {"bound": "hbm", "alg_bytes_per_cell_update": ALG_BYTES_PER_CELL,
                           "achieved": step_bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",

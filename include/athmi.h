@@ -136,15 +136,21 @@ int athmi_set_corr_rows(athmi_grid *g, int rows);
  * are identical. */
 int athmi_set_x2_kind(athmi_grid *g, int kind);
 int athmi_get_x2_kind(const athmi_grid *g, int *kind);
+/* Fusion depth of the step: 0 = the Step 9 corner emf (integrate_emf3_corner,
+ * __integrate_2d__.c:632-702) in its own kernel, written to a plane the final
+ * update reads; 1 = computed inside the final update's tiles. Picked by the
+ * autotuner; results are identical. */
+int athmi_set_fusion(athmi_grid *g, int depth);
+int athmi_get_fusion(const athmi_grid *g, int *depth);
 /* The current launch shape (any pointer may be NULL) */
 int athmi_get_shape(const athmi_grid *g, int *x2_seg, int *persist, int *xcd_remap,
                     int *corr_rows);
 int athmi_use_graph(athmi_grid *g, int on);
 /* Autotuner: time every launch shape (x2 predictor form x x2 marching depth
  * 16..128 x persistent workgroups per CU 0..32 x tile order, then the
- * corrector's tile height
- * 4/2/1 at the best of those) over `steps` main-loop steps from the current
- * state through the step graph, restore the state and clocks; the four
+ * corrector's tile height 4/2/1 and the fusion depth at the best of those)
+ * over `steps` main-loop steps from the current state through the step
+ * graph, restore the state and clocks; the four
  * fastest shapes then run three more trials each, in turn, and the lowest
  * total wins (returned with its mean ms per step; athmi_get_shape gives all
  * of it). No reference counterpart (the reference launches fixed 32-thread

@@ -41,14 +41,19 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("x2_kind", [0, 1])
+# kernel forms: (x2 predictor form, fusion depth); results must not depend on them
+FORMS = [(0, 0), (1, 0), (1, 1)]
+
+
+@pytest.mark.parametrize("forms", FORMS, ids=lambda f: f"x2form{f[0]}_fusion{f[1]}")
 @pytest.mark.parametrize("flux", ["roe", "hlle"])
 @pytest.mark.parametrize("problem,overrides,nsteps", CASES)
-def test_exact_steps_match_oracle(problem, overrides, nsteps, flux, x2_kind):
+def test_exact_steps_match_oracle(problem, overrides, nsteps, flux, forms):
     pkg = load_pkg()
-    say(f"[gpu] exact {problem} {flux} {overrides} steps={nsteps} x2 predictor form {x2_kind}")
+    say(f"[gpu] exact {problem} {flux} {overrides} steps={nsteps} forms {forms}")
     gpu, par, tlim, nlim = pkg.setup_problem(problem, overrides, flux=flux, variant="exact")
-    gpu.set_x2_kind(x2_kind)
+    gpu.set_x2_kind(forms[0])
+    gpu.set_fusion(forms[1])
     ora, tlim_o, _ = oracle_setup(problem, overrides, flux=flux)
     compare_exact(gpu, ora, "initial state")
     gpu.set_limits(tlim, nsteps)
@@ -69,6 +74,7 @@ def test_exact_boundaries_match_oracle(bcs):
           f"grid/ibc_x2={bcs[2]}", f"grid/obc_x2={bcs[3]}", "problem/radius=0.25"]
     say(f"[gpu] exact blast boundaries {bcs}")
     gpu, par, tlim, _ = pkg.setup_problem("blast", ov, variant="exact")
+    gpu.set_fusion(1)
     ora, _, _ = oracle_setup("blast", ov)
     compare_exact(gpu, ora, "initial ghosts")
     gpu.set_limits(tlim, 25)
