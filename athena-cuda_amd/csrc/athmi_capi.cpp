@@ -66,7 +66,7 @@ int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
   g->bl.shape.xcd_remap = 1;
   g->bl.shape.corr_rows = kTileY;
   g->bl.shape.x2_kind = 1;  // the two-wave x2 predictor (1.79 against 2.05 ms at 4096^2)
-  g->bl.shape.fuse_corner = 0;
+  g->bl.shape.fuse_corner = 1;  // Step 9 corners inside k_final (the autotuner's pick at 4096^2)
   // device memory: per instance 2*NSTATE state planes + NSCRATCH scratch planes
   std::vector<size_t> off(n);
   size_t total = 0;

@@ -72,18 +72,21 @@ def pmc_summary():
         return None
 
 
-def cpu_baseline(nx=1024, steps=10):
+def cpu_leg(overrides, steps):
     """The reference's own CPU integrator (oracle/_ref, built from
-    /root/reference/src in the build container) on a bounded sample of the
-    same workload: blast at nx^2 for a few steps, one core."""
+    /root/reference/src in the build container; the oracle restatement where
+    that is absent) on the blast of athinput.blast with these overrides, for
+    a bounded number of main-loop steps, one core, in a child process.
+    Returns (kind, steps, seconds)."""
     from helpers import REF_DIR, ref_available
+    ov = list(overrides)
     if ref_available("roe"):
         code = f"""
 import ctypes, time, json
 L = ctypes.CDLL({os.path.join(REF_DIR, 'libathref_roe.so')!r})
-ov = [b"grid/Nx1={nx}", b"grid/Nx2={nx}"]
-arr = (ctypes.c_char_p * 2)(*ov)
-L.ref_open({os.path.join(ROOT, 'athena-cuda_amd', 'inputs', 'athinput.blast').encode()!r}, 2, arr)
+ov = [o.encode() for o in {ov!r}]
+arr = (ctypes.c_char_p * len(ov))(*ov)
+L.ref_open({os.path.join(ROOT, 'athena-cuda_amd', 'inputs', 'athinput.blast').encode()!r}, len(ov), arr)
 L.ref_setup(0)
 t = time.perf_counter()
 n = L.ref_run(ctypes.c_double(0.2), {steps})
@@ -96,7 +99,7 @@ print(json.dumps(dict(steps=n, seconds=dt)))
 import sys, time, json
 sys.path.insert(0, {os.path.join(ROOT, 'tests')!r})
 from helpers import oracle_setup
-g, tlim, _ = oracle_setup("blast", ["grid/Nx1={nx}", "grid/Nx2={nx}"])
+g, tlim, _ = oracle_setup("blast", {ov!r})
 t = time.perf_counter()
 n = g.run(tlim, {steps})
 dt = time.perf_counter() - t
@@ -105,9 +108,19 @@ print(json.dumps(dict(steps=n, seconds=dt)))
         kind = "port"
     env = dict(os.environ, OMP_NUM_THREADS="1")
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
-                         check=True).stdout.strip().splitlines()[-1]
-    r = json.loads(out)
-    rate = nx * nx * r["steps"] / r["seconds"]
+                         check=True).stdout
+    # the reference's own stdout shares the pipe: take the result line
+    r = json.loads([x for x in out.splitlines() if x.startswith('{"steps"')][-1])
+    return kind, r["steps"], r["seconds"]
+
+
+def cpu_baseline(n=4096, steps=2):
+    """The CPU baseline on a bounded sample of the bench workload itself
+    (blast n^2, `steps` main-loop steps, one core), and beside it the
+    reference's parity configuration (blast 400x600, 100 steps)."""
+    kind, ns, sec = cpu_leg([f"grid/Nx1={n}", f"grid/Nx2={n}"], steps)
+    rate = n * n * ns / sec
+    kind2, ns2, sec2 = cpu_leg([], 100)
     cpu = ""
     try:
         with open("/proc/cpuinfo") as fh:
@@ -118,8 +131,11 @@ print(json.dumps(dict(steps=n, seconds=dt)))
     except OSError:
         pass
     return {"value": rate, "unit": "cell-updates/s", "cores": 1, "kind": kind,
-            "sample": f"blast {nx}x{nx}, {r['steps']} main-loop steps, single thread "
-                      f"({r['seconds']:.1f} s; host {cpu}, nproc {os.cpu_count()})"}
+            "sample": f"blast {n}x{n}, {ns} main-loop steps, single thread ({sec:.1f} s; "
+                      f"host {cpu}, nproc {os.cpu_count()})",
+            "blast_400x600_100_steps": {"value": 400 * 600 * ns2 / sec2,
+                                        "unit": "cell-updates/s", "cores": 1, "kind": kind2,
+                                        "steps": ns2, "seconds": sec2}}
 
 
 def batched_leg(pkg, args, plain_rate):
@@ -415,7 +431,8 @@ def main():
         pkg._check(L.athmi_copy_calibration(1 << 30, 3), "copy calibration")
         out["copy_calibration_bytes"] = {"read": 1 << 30, "write": 1 << 30, "reps": 3}
     if not args.no_cpu:
-        log("[bench] CPU baseline (reference CPU integrator, 1 core)")
+        log("[bench] CPU baseline (reference CPU integrator, 1 core): blast 4096^2 x 2 steps, "
+            "400x600 x 100 steps")
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
 
