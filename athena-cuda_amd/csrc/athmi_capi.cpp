@@ -94,6 +94,12 @@ int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
     I.g1 = d.gamma - 1.0;  // main.c:93-94
     I.g2 = d.gamma - 2.0;
     I.ps = round_up((size_t)I.pitch * I.sy, 64);
+    // the kernels address a plane with 32-bit element offsets (bytes < 2^32)
+    if (I.ps >= ((size_t)1 << 29)) {
+      delete g;
+      return fail(ATHMI_EINVAL, "athmi_create: a plane of (Nx1+8) x (Nx2+8) cells exceeds 2^29 "
+                                "doubles");
+    }
     off[k] = total;
     total += (size_t)(2 * NSTATE + NSCRATCH) * I.ps;
   }

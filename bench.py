@@ -367,6 +367,13 @@ def main():
     ms_step = 1e3 * sec / args.steps
     log(f"[bench] {ms_step:.3f} ms/step, {value / 1e9:.3f} G cell-updates/s")
 
+    # constrained transport: max |div B| over the interior after the timed steps
+    # (the exact variant reproduces the reference's roundoff; B is O(1) here)
+    divb = g.divb_max(0)
+    dx = min(1.0 / n, 1.5 / n)
+    log(f"[bench] max|div B| {divb:.3e} (x dx {divb * dx:.3e}) after {args.warmup + args.steps} "
+        "steps")
+
     # kernel-by-kernel pass with HIP events on the handle's stream
     avg = (ctypes.c_double * 16)()
     pkg._check(L.athmi_profile_steps(g._h, args.profile_steps, avg, 16), "profile")
@@ -409,6 +416,8 @@ def main():
                                                  "xcd_remap": shape[2], "corr_rows": shape[3],
                                                  "x2_form": shape[5], "fusion": shape[6]}},
         "roofline": roofline,
+        "divb": {"max_abs": divb, "max_abs_times_dx": divb * dx,
+                 "steps": args.warmup + args.steps},
         "step_roofline": {"bound": "hbm", "alg_bytes_per_cell_update": ALG_BYTES_PER_CELL,
                           "achieved": step_bw, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_bw / HBM_PEAK_GBS},
