@@ -57,6 +57,7 @@ SIGNATURES = {
     "athmi_get_clock": (_I, [_P, _I, _DP, _DP, _IP]),
     "athmi_set_limits": (_I, [_P, _I, _D, _I]),
     "athmi_get_failure": (_I, [_P, _I, _IP, _IP]),
+    "athmi_set_fail_stop": (_I, [_P, _I, _I]),
     "athmi_set_pause": (_I, [_P, _I, _D]),
     "athmi_set_bvals": (_I, [_P]),
     "athmi_new_dt": (_I, [_P]),
@@ -75,6 +76,8 @@ SIGNATURES = {
     "athmi_get_x2_kind": (_I, [_P, _IP]),
     "athmi_set_fusion": (_I, [_P, _I]),
     "athmi_get_fusion": (_I, [_P, _IP]),
+    "athmi_set_strip": (_I, [_P, _I]),
+    "athmi_get_strip": (_I, [_P, _IP]),
     "athmi_get_shape": (_I, [_P, _IP, _IP, _IP, _IP]),
     "athmi_device_state": (_I, [_P, _I, ctypes.POINTER(_P), _IP]),
     "athmi_pgen_blast": (_I, [_P, _I, _D, _D, _D, _D, _D, _D, _D]),
@@ -234,12 +237,18 @@ class Grid:
 
     def failure(self, inst=0):
         """(flags, nstep): the FAIL_* bits raised by the states this instance
-        produced and the step of the first one (-1: none). A failed instance
-        no longer steps."""
+        produced and the step of the first one (-1: none). An instance whose
+        flags meet its stop set (set_fail_stop) no longer steps."""
         f, n = ctypes.c_int(), ctypes.c_int()
         _check(lib().athmi_get_failure(self._h, inst, ctypes.byref(f), ctypes.byref(n)),
                "get_failure")
         return f.value, n.value
+
+    def set_fail_stop(self, mask, inst=-1):
+        """FAIL_* bits that stop the main-loop step (default FAIL_NONFINITE |
+        FAIL_DENSITY; a pressure-floor hit is only reported, as the reference
+        floors and steps on)"""
+        _check(lib().athmi_set_fail_stop(self._h, inst, int(mask)), "set_fail_stop")
 
     def set_limits(self, tlim, nlim=-1, inst=-1):
         _check(lib().athmi_set_limits(self._h, inst, tlim, nlim), "set_limits")
@@ -304,12 +313,24 @@ class Grid:
         return k.value
 
     def set_fusion(self, depth):
-        """0: corner emf of Step 9 in its own kernel, 1: inside the final kernel"""
+        """0: the corner emf of Steps 4 and 9 in kernels of their own; 1: the Step 9
+        corners inside the final kernel; 2: also the Step 4 corners inside the x1
+        corrector"""
         _check(lib().athmi_set_fusion(self._h, depth), "set_fusion")
 
     def fusion(self):
         k = ctypes.c_int()
         _check(lib().athmi_get_fusion(self._h, ctypes.byref(k)), "get_fusion")
+        return k.value
+
+    def set_strip(self, tile_cols):
+        """Tile order of the corrector and final kernels: 0 row by row, s > 0
+        strips of s tile columns"""
+        _check(lib().athmi_set_strip(self._h, int(tile_cols)), "set_strip")
+
+    def strip(self):
+        k = ctypes.c_int()
+        _check(lib().athmi_get_strip(self._h, ctypes.byref(k)), "get_strip")
         return k.value
 
     def set_tile_order(self, xcd_remap=1):

@@ -60,13 +60,14 @@ int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
   // faces marched per lane by the x2 predictor: enough 64-lane workgroups
   // (about 2048, two per SIMD) to fill the chip, 4..16 faces each
   long cols = 0;
-  for (int k = 0; k < n; k++) cols += (long)cdiv(desc[k].nx1 + 5, kX2Block) * (desc[k].nx2 + 4);
+  for (int k = 0; k < n; k++) cols += (long)cdiv(desc[k].nx1 + 6, kX2Block) * (desc[k].nx2 + 4);
   g->bl.shape.x2_seg = (int)std::max(4L, std::min(16L, cols / 2048));
   g->bl.shape.persist = 0;
   g->bl.shape.xcd_remap = 1;
   g->bl.shape.corr_rows = kTileY;
   g->bl.shape.x2_kind = 1;  // the two-wave x2 predictor (1.79 against 2.05 ms at 4096^2)
   g->bl.shape.fuse_corner = 1;  // Step 9 corners inside k_final (the autotuner's pick at 4096^2)
+  g->bl.shape.strip = 0;        // corrector and final tiles row by row
   // device memory: per instance 2*NSTATE state planes + NSCRATCH scratch planes
   std::vector<size_t> off(n);
   size_t total = 0;
@@ -126,6 +127,9 @@ int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
     hc[k].tpause = HUGE_VAL;
     hc[k].nlim = -1;
     hc[k].fail_nstep = -1;
+    // the reference floors the pressure and steps on (convert_var.c:49): a
+    // floor hit is reported, only a non-finite value or d <= 0 stops
+    hc[k].fail_stop = ATHMI_FAIL_NONFINITE | ATHMI_FAIL_DENSITY;
     Inst& I = g->hinst[k];
     I.state[0] = g->dmem + off[k];
     I.state[1] = I.state[0] + NSTATE * I.ps;
@@ -267,6 +271,22 @@ int athmi_get_failure(athmi_grid* g, int inst, int* flags, int* nstep) {
   return ATHMI_OK;
 }
 
+int athmi_set_fail_stop(athmi_grid* g, int inst, int mask) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  if (mask & ~(ATHMI_FAIL_NONFINITE | ATHMI_FAIL_DENSITY | ATHMI_FAIL_PRESSURE))
+    return fail(ATHMI_EINVAL, "athmi_set_fail_stop: unknown failure bit");
+  const int lo = inst < 0 ? 0 : inst, hi = inst < 0 ? g->n - 1 : inst;
+  if (inst >= g->n) return fail(ATHMI_EINVAL, "instance index out of range");
+  for (int k = lo; k <= hi; k++) {
+    Clock c;
+    int rc = g->read_clock(k, c);
+    if (rc) return rc;
+    c.fail_stop = mask;
+    if ((rc = g->write_clock_head(k, c))) return rc;
+  }
+  return ATHMI_OK;
+}
+
 int athmi_set_limits(athmi_grid* g, int inst, double tlim, int nlim) {
   if (!g) return fail(ATHMI_EINVAL, "null handle");
   const int lo = inst < 0 ? 0 : inst, hi = inst < 0 ? g->n - 1 : inst;
@@ -353,7 +373,7 @@ int athmi_run(athmi_grid* g, int max_steps, int* steps) {
     bool running = false;
     for (const Clock& c : hc)
       if (c.time < c.tlim && (c.nlim < 0 || c.nstep < c.nlim) && c.time < c.tpause &&
-          c.fail == 0)
+          (c.fail & c.fail_stop) == 0)
         running = true;
     if (!running) break;
     int m = chunk;
@@ -530,12 +550,22 @@ int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd
     sh.corr_rows = rows;
     rc = sweep(sh);
   }
-  // fusion depth: the Step 9 corner emf in its own kernel or inside the
-  // final kernel's tiles
+  // tile order of the corrector and final kernels: whole rows or strips
+  const LaunchShape base1 = best_shape;
+  for (int strip : {0, 2, 4, 8}) {
+    if (rc || strip == base1.strip) continue;
+    LaunchShape sh = base1;
+    sh.strip = strip;
+    rc = sweep(sh);
+  }
+  // fusion depth: the corner emf of Steps 4 and 9 in kernels of their own
+  // (0), the Step 9 corners inside the final kernel's tiles (1), and also the
+  // Step 4 corners inside the x1 corrector's tiles (2)
   const LaunchShape base2 = best_shape;
-  if (!rc) {
+  for (int depth = 0; !rc && depth <= 2; depth++) {
+    if (depth == base2.fuse_corner) continue;
     LaunchShape sh = base2;
-    sh.fuse_corner = base2.fuse_corner ? 0 : 1;
+    sh.fuse_corner = depth;
     rc = sweep(sh);
   }
   // single short trials differ by about 1% from run to run: the fastest few
@@ -586,10 +616,25 @@ int athmi_set_x2_kind(athmi_grid* g, int kind) {
 
 int athmi_set_fusion(athmi_grid* g, int depth) {
   if (!g) return fail(ATHMI_EINVAL, "null handle");
-  if (depth != 0 && depth != 1) return fail(ATHMI_EINVAL, "athmi_set_fusion: depth must be 0 or 1");
+  if (depth < 0 || depth > 2)
+    return fail(ATHMI_EINVAL, "athmi_set_fusion: depth must be 0, 1 or 2");
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->bl.shape.fuse_corner = depth;
   return g->refresh_tiles();
+}
+
+int athmi_set_strip(athmi_grid* g, int tile_cols) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  if (tile_cols < 0) return fail(ATHMI_EINVAL, "athmi_set_strip: tile_cols must be >= 0");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->bl.shape.strip = tile_cols;
+  return g->refresh_tiles();
+}
+
+int athmi_get_strip(const athmi_grid* g, int* tile_cols) {
+  if (!g || !tile_cols) return fail(ATHMI_EINVAL, "athmi_get_strip: null argument");
+  *tile_cols = g->bl.shape.strip;
+  return ATHMI_OK;
 }
 
 int athmi_get_fusion(const athmi_grid* g, int* depth) {

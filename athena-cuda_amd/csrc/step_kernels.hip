@@ -60,6 +60,26 @@ __device__ __forceinline__ int xcd_tile(int t, int n, int remap) {
   return x < rem ? x * (per + 1) + q : rem * (per + 1) + (x - rem) * per + q;
 }
 
+// Position (tx, ty) of tile `local` in an ntx x nty grid of 2-D tiles. strip 0:
+// row by row. strip s > 0: strips of s tile columns, each walked row by row
+// before the next strip, so that the tile dispatched before a tile is mostly
+// the one above it (s tiles back), whose rows -- this tile's upper halo -- the
+// same XCD's L2 has just fetched. Any bijection is correct.
+__device__ __forceinline__ void tile_pos(int local, int ntx, int nty, int strip, int& tx,
+                                         int& ty) {
+  if (strip <= 0 || strip >= ntx) {
+    ty = local / ntx;
+    tx = local - ty * ntx;
+    return;
+  }
+  const int band = strip * nty;
+  const int sidx = local / band;
+  const int w = min(strip, ntx - sidx * strip);  // the last strip may be narrower
+  const int r = local - sidx * band;
+  ty = r / w;
+  tx = sidx * strip + (r - ty * w);
+}
+
 // instance owning global tile t of kernel kind `kind` (tile0 ascending)
 __device__ __forceinline__ int find_inst(const Inst* __restrict__ in, int n, int kind, int t) {
   int lo = 0, hi = n - 1;
@@ -272,9 +292,11 @@ __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ i
     const Clock* clk = g.clk;
     if (!clk->active || g.flux != FLUX) continue;
     const int local = t - g.tile0[K_PRED_X2];
-    const int ncb = dcdiv(g.nx1 + 5, kX2Block);
+    // chunks from column 0: each row segment starts on a cache line
+    const int ncb = dcdiv(g.nx1 + 6, kX2Block);
     const int sg = local / ncb, cb = local - sg * ncb;
-    const int i = g.is - 2 + cb * kX2Block + lane;
+    const int i = cb * kX2Block + lane;
+    if (i < g.is - 2) continue;
     if (i > g.ie + 2) continue;
     const int jf0 = g.js - 1 + sg * seg_len;
     const int jf1 = min(jf0 + seg_len - 1, g.je + 2);
@@ -363,14 +385,26 @@ __global__ void __launch_bounds__(kX2Block) k_pred_x2(const Inst* __restrict__ i
 // slot, PLM of cell j (its Wl(j+1) overwrites the dead row j-1), then the face
 // state build and the flux reading Wl(j), Wr(j) back from LDS. The slots
 // rotate: row j-1's slot becomes the next pending Wl, Wl(j)'s the next free.
-// LDS: 5 x 7 x 64 doubles = 17.5 KiB per wave (eight waves per CU).
+// The face-normal field and the 1/2 dt source term come from LDS as well
+// (five doubles per lane): B2i of rows j and j+1, M1/d of the newest row,
+// and the source s(r) = (M1/d)(r) (B2i(r+1) - B2i(r)) / dx2 of rows j-1 and
+// j -- the one product both faces of cell r add (to Wr(r) and Wl(r+1)), so
+// each row's M1/d and B2i are read and divided once.
+// LDS: (5 x 7 + 5) x 64 doubles = 20 KiB per wave: eight waves fill a CU's
+// 160 KiB.
 constexpr int kX2Slots = 5;  // 4 rotating + the Wr slot
+constexpr int kX2Aux = 5;    // B2i ring (2), s ring (2), M1/d carry
 template <bool SAVE, int FLUX>
 __global__ void __launch_bounds__(kX2Block, 2) k_pred_x2l(const Inst* __restrict__ insts,
                                                           int ninst, int ntiles, int seg_len,
                                                           int remap) {
   __shared__ double slots[kX2Slots][7][kX2Block];
+  __shared__ double aux[kX2Aux][kX2Block];
   const int lane = threadIdx.x;
+  // aux rows: B2i of row r at r & 1, s(r) at 2 + (r & 1), M1/d of the newest row
+  auto b2i_slot = [&](int r) -> double& { return aux[r & 1][lane]; };
+  auto src_slot = [&](int r) -> double& { return aux[2 + (r & 1)][lane]; };
+  double& v1_carry = aux[4][lane];
   auto put = [&](int s, const Prim7& w) {
     slots[s][0][lane] = w.d; slots[s][1][lane] = w.vx; slots[s][2][lane] = w.vy;
     slots[s][3][lane] = w.vz; slots[s][4][lane] = w.p; slots[s][5][lane] = w.by;
@@ -397,9 +431,11 @@ __global__ void __launch_bounds__(kX2Block, 2) k_pred_x2l(const Inst* __restrict
     const Clock* clk = g.clk;
     if (!clk->active || g.flux != FLUX) continue;
     const int local = uni(t - g.tile0[K_PRED_X2]);
-    const int ncb = dcdiv(uni(g.nx1) + 5, kX2Block);
+    // chunks from column 0: each row segment starts on a cache line
+    const int ncb = dcdiv(uni(g.nx1) + 6, kX2Block);
     const int sg = local / ncb, cb = local - sg * ncb;
-    const int i = uni(g.is) - 2 + cb * kX2Block + lane;
+    const int i = cb * kX2Block + lane;
+    if (i < uni(g.is) - 2) continue;
     if (i > uni(g.ie) + 2) continue;
     const int jf0 = uni(g.js) - 1 + sg * seg_len;
     const int jf1 = min(jf0 + seg_len - 1, uni(g.je) + 2);
@@ -413,18 +449,29 @@ __global__ void __launch_bounds__(kX2Block, 2) k_pred_x2l(const Inst* __restrict
     double* __restrict__ pw = uni(g.prims);
     const size_t ps = (size_t)uni((int)g.ps);
     const int pitch = uni(g.pitch);
-    // cons->prim of row jj in the permuted order (d, M2, M3, M1, E, B3c, B1c), Bx = B2c
-    auto prim_at = [&](int jj) -> Prim7 {
+    // cons->prim of row jj in the permuted order (d, M2, M3, M1, E, B3c, B1c), Bx = B2c;
+    // with M1/d (the source term's division form) and B2i of the row
+    auto load_row = [&](int jj, int slot, double& v1, double& b2i) {
       const uint32_t k = (uint32_t)(jj * pitch + i);
-      const Cons7 u{LD(st + F_D * ps, k), LD(st + F_M2 * ps, k), LD(st + F_M3 * ps, k), LD(st + F_M1 * ps, k),
+      const double d = LD(st + F_D * ps, k), m1 = LD(st + F_M1 * ps, k);
+      const Cons7 u{d, LD(st + F_M2 * ps, k), LD(st + F_M3 * ps, k), m1,
                     LD(st + F_E * ps, k), LD(st + F_B3C * ps, k), LD(st + F_B1C * ps, k)};
-      return cons_to_prim(u, LD(st + F_B2C * ps, k), eos);
+      put(slot, cons_to_prim(u, LD(st + F_B2C * ps, k), eos));
+      v1 = m1 / d;
+      b2i = LD(st + F_B2I * ps, k);
     };
     auto bx_at = [&](int jj) { return LD(st + F_B2C * ps, (uint32_t)(jj * pitch + i)); };
-    // start of the march: rows jf0-2, jf0-1, jf0 and the PLM of cell jf0-1
-    put(0, prim_at(jf0 - 2));
-    put(1, prim_at(jf0 - 1));
-    put(2, prim_at(jf0));
+    // start of the march: rows jf0-2, jf0-1, jf0, the source s(jf0-1) and
+    // the PLM of cell jf0-1
+    {
+      double v1m, b2m, v10, b20, v1u, b2u;
+      load_row(jf0 - 2, 0, v1u, b2u);
+      load_row(jf0 - 1, 1, v1m, b2m);
+      load_row(jf0, 2, v10, b20);
+      src_slot(jf0 - 1) = v1m * (b20 - b2m) / dx2;
+      b2i_slot(jf0) = b20;
+      v1_carry = v10;
+    }
     {
       Prim7 wl, wr_unused;
       plm_cell_s(LdsStencil{&get, 0, 1, 2}, bx_at(jf0 - 1), dtodx2, eos, wl, wr_unused);
@@ -434,7 +481,14 @@ __global__ void __launch_bounds__(kX2Block, 2) k_pred_x2l(const Inst* __restrict
     int sP = 1, sQ = 2, sW = 0, sF = 3;  // rows j-1, j; pending Wl(j); free
     for (int j = jf0; j <= jf1; j++) {
       phase_fence();
-      put(sF, prim_at(j + 1));
+      {
+        // row j+1; then s(j) = (M1/d)(j) (B2i(j+1) - B2i(j)) / dx2 (:216-224)
+        double v1n, b2n;
+        load_row(j + 1, sF, v1n, b2n);
+        src_slot(j) = v1_carry * (b2n - b2i_slot(j)) / dx2;
+        b2i_slot(j + 1) = b2n;
+        v1_carry = v1n;
+      }
       phase_fence();
       {
         Prim7 wl_next, wr_cur;
@@ -448,17 +502,14 @@ __global__ void __launch_bounds__(kX2Block, 2) k_pred_x2l(const Inst* __restrict
       const uint32_t k = (uint32_t)(j * pitch + i);
       const int sw = sW;
       // face states: PLM output + MHD source terms for 1/2 dt (:216-224) +
-      // prim->cons (:235-236); rebuilt from LDS and memory for the HLLE fallback
+      // prim->cons (:235-236); rebuilt from LDS for the HLLE fallback
       auto build = [&](Cons7& ul, Cons7& ur, Prim7& wl, Prim7& wr, double& b0) {
         compiler_fence();
         wl = get(sw);
         wr = get(kWr);
-        const double bm = LD(st + F_B2I * ps, k - pitch), bp = LD(st + F_B2I * ps, k + pitch);
-        b0 = LD(st + F_B2I * ps, k);
-        double src = (LD(st + F_M1 * ps, k - pitch) / LD(st + F_D * ps, k - pitch)) * (b0 - bm) / dx2;
-        wl.bz = wl.bz + hdt * src;
-        src = (LD(st + F_M1 * ps, k) / LD(st + F_D * ps, k)) * (bp - b0) / dx2;
-        wr.bz = wr.bz + hdt * src;
+        b0 = b2i_slot(j);
+        wl.bz = wl.bz + hdt * src_slot(j - 1);
+        wr.bz = wr.bz + hdt * src_slot(j);
         ul = prim_to_cons(wl, b0, eos);
         ur = prim_to_cons(wr, b0, eos);
       };
@@ -555,10 +606,18 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_corner(const Inst* __restric
 // FLUX: the Riemann solver; a launch takes only the tiles of instances that use
 // it. With one solver per kernel (the Roe one keeps its HLLE fallback) the
 // corrector fits two waves per SIMD.
-template <int PART, int TY, int FLUX>  // TY: rows of a tile of kTileX x TY faces
+// CP (x1 part only): the tile first computes the Step 4 corner emf of its rows
+// j0..j0+TY and columns i0..i0+kTileX into LDS from the predictor fluxes, and
+// stores the corners it owns -- together the range [js-1,je+2]x[is-1,ie+2] --
+// in the corner plane for the x2 part (no k_corner<pred> launch).
+constexpr int kCornerRows = kTileY + 1, kCornerCols = kTileX + 1;
+template <int PART, int TY, int FLUX, bool CP = false>  // TY: rows of a tile of kTileX x TY faces
 __global__ void __launch_bounds__(kTileX* TY, 2) k_corr(const Inst* __restrict__ insts,
-                                                            int ninst, int ntiles, int remap) {
+                                                            int ninst, int ntiles, int remap,
+                                                            int strip) {
+  static_assert(!CP || PART == 1, "the corner phase belongs to the x1 part");
   __shared__ double park_lds[14 * kTileX * TY];
+  __shared__ double cemf[CP ? (TY + 1) * kCornerCols : 1];
   const LdsPark park{park_lds, kTileX * TY, (int)(threadIdx.y * kTileX + threadIdx.x)};
   for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
     const int t = xcd_tile(t0, ntiles, remap);
@@ -567,10 +626,35 @@ __global__ void __launch_bounds__(kTileX* TY, 2) k_corr(const Inst* __restrict__
     if (!clk->active || g.flux != FLUX) continue;
     const int local = uni(t - g.tile0[K_CORR]);
     const int gis = uni(g.is), gjs = uni(g.js);
-    const int ntx = dcdiv(uni(g.nx1) + 3, kTileX);
-    const int ty = local / ntx, tx = local - ty * ntx;
-    const int i = gis - 1 + tx * kTileX + threadIdx.x;
-    const int j = gjs - 1 + ty * TY + threadIdx.y;
+    int tx, ty;
+    tile_pos(local, dcdiv(uni(g.nx1) + 3, kTileX), dcdiv(uni(g.nx2) + 3, TY), strip, tx, ty);
+    const int i0 = gis - 1 + tx * kTileX, j0 = gjs - 1 + ty * TY;
+    const int i = i0 + threadIdx.x;
+    const int j = j0 + threadIdx.y;
+    if (CP) {
+      // every thread of the block reaches both barriers: the one before the
+      // fill keeps the previous tile's readers ahead of it
+      __syncthreads();
+      double* __restrict__ scw = uni(g.scratch);
+      const size_t psw = (size_t)uni((int)g.ps);
+      const int pw = uni(g.pitch);
+      const int je2 = uni(g.je) + 2, ie2 = uni(g.ie) + 2;
+      for (int c = threadIdx.y * kTileX + threadIdx.x; c < (TY + 1) * kCornerCols;
+           c += kTileX * TY) {
+        const int cr = c / kCornerCols, cl = c - cr * kCornerCols;
+        const int cj = j0 + cr, ci = i0 + cl;
+        double e = 0.0;
+        if (cj <= je2 && ci <= ie2) {
+          e = corner_emf(scw + S_F1 * psw, scw + S_F2 * psw, scw + S_ECC * psw, psw, pw, cj, ci);
+          // owned: the tile's own rows and columns, and the last row and
+          // column of the range where they fall just outside them
+          if ((cr < TY || cj == je2) && (cl < kTileX || ci == ie2))
+            ST(scw + S_EMF * psw, (uint32_t)(cj * pw + ci), e);
+        }
+        cemf[c] = e;
+      }
+      __syncthreads();
+    }
     if (i > uni(g.ie) + 1 || j > uni(g.je) + 1) continue;
     const double dt = uni(clk->dt);
     const double dtodx1 = dt / uni(g.dx1), dtodx2 = dt / uni(g.dx2);
@@ -629,7 +713,12 @@ __global__ void __launch_bounds__(kTileX* TY, 2) k_corr(const Inst* __restrict__
           R.bz = R.bz + hdtodx1 * v3 * dbx;
           R.e = R.e + hdtodx1 * b3 * v3 * dbx;
         }
-        bx = b1h(k);
+        if (CP) {
+          const int cc = threadIdx.y * kCornerCols + threadIdx.x;
+          bx = LD(B1i, k) - hdtodx2 * (cemf[cc + kCornerCols] - cemf[cc]);
+        } else {
+          bx = b1h(k);
+        }
         wl = cons_to_prim(L, bx, eos);
         wr = cons_to_prim(R, bx, eos);
       };
@@ -742,10 +831,10 @@ __device__ __forceinline__ int cell_fail_bits(const Inst& g, double d, double M1
 // x (kTileX+1) corners -- into LDS from the corrector fluxes, instead of
 // reading the plane k_corner<false> wrote (one launch and the corner plane's
 // write and read fewer).
-constexpr int kCornerRows = kTileY + 1, kCornerCols = kTileX + 1;
 template <bool FUSED>
 __global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict__ insts,
-                                                           int ninst, int ntiles, int remap) {
+                                                           int ninst, int ntiles, int remap,
+                                                           int strip) {
   __shared__ double red[4];
   __shared__ double cemf[FUSED ? kCornerRows * kCornerCols : 1];
   for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
@@ -754,8 +843,8 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict
     Clock* clk = g.clk;
     if (!clk->active) continue;
     const int local = t - g.tile0[K_FINAL];
-    const int ntx = dcdiv(g.nx1 + 1, kTileX);
-    const int ty = local / ntx, tx = local - ty * ntx;
+    int tx, ty;
+    tile_pos(local, dcdiv(g.nx1 + 1, kTileX), dcdiv(g.nx2 + 1, kTileY), strip, tx, ty);
     const int i0 = g.is + tx * kTileX, j0 = g.js + ty * kTileY;
     const int i = i0 + threadIdx.x;
     const int j = j0 + threadIdx.y;
@@ -890,7 +979,7 @@ __global__ void __launch_bounds__(kGhostBlock) k_ghost(const Inst* __restrict__ 
 // ---------------------------------------------------------------------------
 // Clock kernels: one workgroup of 256 threads per instance.
 // mode 0: main-loop step (main.c:221-232): run while time < tlim and
-//         (nlim < 0 or nstep < nlim) and no failure bit is set (FailBits);
+//         (nlim < 0 or nstep < nlim) and no failure bit of fail_stop is set;
 //         clamp dt to reach tlim exactly. The
 //         step also waits (does not run) once time >= tpause, so that the
 //         host can write the outputs due at the top of the iteration
@@ -903,7 +992,7 @@ __global__ void __launch_bounds__(256) k_begin(const Inst* __restrict__ insts, i
     int act = 1;
     if (mode == 0) {
       act = (c->time < c->tlim) && (c->nlim < 0 || c->nstep < c->nlim) &&
-            (c->time < c->tpause) && (c->fail == 0);
+            (c->time < c->tpause) && ((c->fail & c->fail_stop) == 0);
       if (act && (c->tlim - c->time) < c->dt) c->dt = (c->tlim - c->time);
     }
     c->active = act;
@@ -1024,19 +1113,21 @@ static PredX2Kernel pred_x2_kernel(bool save, int flux, int kind) {
   if (flux == kFluxHlle) return save ? k_pred_x2<true, kFluxHlle> : k_pred_x2<false, kFluxHlle>;
   return save ? k_pred_x2<true, kFluxRoe> : k_pred_x2<false, kFluxRoe>;
 }
-typedef void (*CorrKernel)(const Inst*, int, int, int);
+typedef void (*CorrKernel)(const Inst*, int, int, int, int);
 // the corrector instantiation for a part (1: x1 faces, 2: x2 faces and Step
 // 7), tile height and Riemann solver
-template <int PART>
+template <int PART, bool CP>
 static CorrKernel corr_kernel_part(int ty, int flux) {
   if (flux == kFluxHlle)
-    return ty == 1 ? k_corr<PART, 1, kFluxHlle> : ty == 2 ? k_corr<PART, 2, kFluxHlle>
-                                                            : k_corr<PART, 4, kFluxHlle>;
-  return ty == 1 ? k_corr<PART, 1, kFluxRoe> : ty == 2 ? k_corr<PART, 2, kFluxRoe>
-                                                         : k_corr<PART, 4, kFluxRoe>;
+    return ty == 1 ? k_corr<PART, 1, kFluxHlle, CP>
+                   : ty == 2 ? k_corr<PART, 2, kFluxHlle, CP> : k_corr<PART, 4, kFluxHlle, CP>;
+  return ty == 1 ? k_corr<PART, 1, kFluxRoe, CP>
+                 : ty == 2 ? k_corr<PART, 2, kFluxRoe, CP> : k_corr<PART, 4, kFluxRoe, CP>;
 }
-static CorrKernel corr_kernel(int part, int ty, int flux) {
-  return part == 1 ? corr_kernel_part<1>(ty, flux) : corr_kernel_part<2>(ty, flux);
+// cp: the x1 part computes the Step 4 corners itself (fusion depth 2)
+static CorrKernel corr_kernel(int part, int ty, int flux, bool cp) {
+  if (part == 2) return corr_kernel_part<2, false>(ty, flux);
+  return cp ? corr_kernel_part<1, true>(ty, flux) : corr_kernel_part<1, false>(ty, flux);
 }
 
 // persistent grids: persist x per_cu workgroups per CU (persist 0: one per tile)
@@ -1090,6 +1181,8 @@ hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
       }
       break;
     case SK_CORNER_P:
+      // at fusion depth 2 the x1 corrector computes these corners
+      if (b.shape.fuse_corner >= 2) break;
       hipLaunchKernelGGL(k_corner<true>, dim3(grid_of(b, K_CORNER_P, 8)), b2, 0, s, b.insts,
                          b.ninst, b.ntiles[K_CORNER_P], b.shape.xcd_remap);
       break;
@@ -1102,9 +1195,9 @@ hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
       const bool x1 = (kernel == SK_CORR);
       for (int f = 0; f < 2; f++) {
         if (!(b.flux_mask & (1 << f))) continue;
-        const CorrKernel fn = corr_kernel(x1 ? 1 : 2, ty, f);
+        const CorrKernel fn = corr_kernel(x1 ? 1 : 2, ty, f, b.shape.fuse_corner >= 2);
         hipLaunchKernelGGL(fn, grid, blk, 0, s, b.insts, b.ninst, b.ntiles[K_CORR],
-                           b.shape.xcd_remap);
+                           b.shape.xcd_remap, b.shape.strip);
       }
       break;
     }
@@ -1117,7 +1210,7 @@ hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
     case SK_FINAL:
       hipLaunchKernelGGL(b.shape.fuse_corner ? k_final<true> : k_final<false>,
                          dim3(grid_of(b, K_FINAL, 8)), b2, 0, s, b.insts, b.ninst,
-                         b.ntiles[K_FINAL], b.shape.xcd_remap);
+                         b.ntiles[K_FINAL], b.shape.xcd_remap, b.shape.strip);
       break;
     case SK_GHOST_CUR:
     case SK_GHOST_NEXT:

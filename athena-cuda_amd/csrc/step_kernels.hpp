@@ -51,7 +51,8 @@ struct Clock {
   int nstep, nlim, active, cur;  // cur: which ping-pong state buffer holds the state
   int fail;        // FailBits of the states produced so far (0: healthy)
   int fail_nstep;  // nstep of the first state that raised a bit (-1: none)
-  int steps_run, pad0;
+  int fail_stop;   // FailBits that stop the main-loop step (the others are only reported)
+  int steps_run;
   unsigned long long divb_bits;           // max |div B| (bit pattern), diagnostics
   unsigned long long dti_slot[kDtSlots];  // bit patterns of non-negative doubles
 };
@@ -73,7 +74,9 @@ struct Inst {
 // tiles_of() below so host and device agree on the decomposition.
 enum KernelKind {
   K_PRED_X1 = 0,  // rows js-2..je+2, faces is-1..ie+2 in segments of kX1Faces
-  K_PRED_X2,      // columns is-2..ie+2 in chunks of kX2Block, faces js-1..je+2 in segments
+  K_PRED_X2,      // columns is-2..ie+2 in chunks of kX2Block from column 0 (row segments
+                  // start on a 128-byte line: the pitch is a multiple of 16), faces
+                  // js-1..je+2 in segments
   K_CORR,         // cells [js-1,je+1] x [is-1,ie+1]
   K_CORNER_P,     // corners [js-1,je+2] x [is-1,ie+2]
   K_CORNER_C,     // corners [js,je+1] x [is,ie+1]
@@ -102,7 +105,7 @@ inline int tiles_of(int k, int nx1, int nx2, int x2seg, int corr_rows) {
   const int sx = nx1 + 8;
   switch (k) {
     case K_PRED_X1: return (nx2 + 4) * cdiv(nx1 + 4, kX1Faces);
-    case K_PRED_X2: return cdiv(nx1 + 5, kX2Block) * cdiv(nx2 + 4, x2seg);
+    case K_PRED_X2: return cdiv(nx1 + 6, kX2Block) * cdiv(nx2 + 4, x2seg);
     case K_CORR: return cdiv(nx1 + 3, kTileX) * cdiv(nx2 + 3, corr_rows);
     case K_CORNER_P: return cdiv(nx1 + 4, kTileX) * cdiv(nx2 + 4, kTileY);
     case K_CORNER_C: return cdiv(nx1 + 2, kTileX) * cdiv(nx2 + 2, kTileY);
@@ -120,8 +123,12 @@ struct LaunchShape {
   int corr_rows;     // rows of a corrector tile (kTileX x corr_rows faces): 1, 2 or 4
   int x2_kind;       // x2 predictor: 0 = stencil marched in registers (one wave per
                      // SIMD), 1 = stencil marched in LDS slots (two waves per SIMD)
-  int fuse_corner;   // 1: the Step 9 corner emf is computed inside the final kernel's
-                     // tiles (no k_corner<corr> launch, no corner plane round trip)
+  int fuse_corner;   // fusion depth. 1: the Step 9 corner emf is computed inside the
+                     // final kernel's tiles (no k_corner<corr> launch, no corner plane
+                     // round trip); 2: also the Step 4 corner emf inside the x1
+                     // corrector's tiles (no k_corner<pred> launch)
+  int strip;         // tile order of the corrector and final kernels: 0 = row by row,
+                     // s > 0 = strips of s tile columns walked row by row
 };
 
 // Everything a launcher needs for one batch (device descriptor array, tile

@@ -81,16 +81,22 @@ int athmi_download(athmi_grid *g, int inst, double *gas, double *b1i, double *b2
 int athmi_set_clock(athmi_grid *g, int inst, double dt, double time, int nstep);
 int athmi_get_clock(athmi_grid *g, int inst, double *dt, double *time, int *nstep);
 /* Failure word of one instance: the step checks every state it produces and
- * raises ATHMI_FAIL_* bits; from then on the main-loop step (athmi_step,
- * athmi_run) no longer runs on that instance, the others go on. *nstep gets
- * the step number of the first state that raised a bit (-1: none). The
- * reference has no such check (it floors pressure, convert_var.c:49 and
- * new_dt.c:54-55, and steps on); athmi_upload of a new state clears the word.
+ * raises ATHMI_FAIL_* bits. Once a bit of the instance's stop set is raised,
+ * the main-loop step (athmi_step, athmi_run) no longer runs on that instance;
+ * the others go on. *nstep gets the step number of the first state that
+ * raised a bit (-1: none). The reference has no such check: it floors the
+ * pressure (convert_var.c:49, new_dt.c:54-55) and steps on, so the default
+ * stop set is NONFINITE | DENSITY (the states from which no further step
+ * means anything) and a pressure-floor hit is only reported -- the run goes on
+ * exactly as the reference's. athmi_upload of a new state clears the word.
  * Either pointer may be NULL. */
 #define ATHMI_FAIL_NONFINITE 1 /* NaN or infinity in d, M1..M3, E or cell-centred B */
 #define ATHMI_FAIL_DENSITY 2   /* d <= 0 */
 #define ATHMI_FAIL_PRESSURE 4  /* Gamma_1 (E - KE - ME) < TINY_NUMBER (the floor engages) */
 int athmi_get_failure(athmi_grid *g, int inst, int *flags, int *nstep);
+/* The stop set of one instance (inst = -1: all), any OR of ATHMI_FAIL_*;
+ * 0 never stops. No reference counterpart. */
+int athmi_set_fail_stop(athmi_grid *g, int inst, int mask);
 /* main-loop limits (main.c:217-218: tlim, nlim; nlim < 0 = none); inst = -1: all */
 int athmi_set_limits(athmi_grid *g, int inst, double tlim, int nlim);
 /* Output cadence (main.c:226 calls data_output, output.c:362-376, at the top
@@ -136,19 +142,28 @@ int athmi_set_corr_rows(athmi_grid *g, int rows);
  * are identical. */
 int athmi_set_x2_kind(athmi_grid *g, int kind);
 int athmi_get_x2_kind(const athmi_grid *g, int *kind);
-/* Fusion depth of the step: 0 = the Step 9 corner emf (integrate_emf3_corner,
- * __integrate_2d__.c:632-702) in its own kernel, written to a plane the final
- * update reads; 1 = computed inside the final update's tiles. Picked by the
- * autotuner; results are identical. */
+/* Fusion depth of the step: 0 = the corner emf of Steps 4 and 9
+ * (integrate_emf3_corner, __integrate_2d__.c:632-702) each in its own kernel,
+ * written to a plane the next kernel reads; 1 = the Step 9 corners computed
+ * inside the final update's tiles; 2 = also the Step 4 corners inside the x1
+ * corrector's tiles. Picked by the autotuner; results are identical. */
 int athmi_set_fusion(athmi_grid *g, int depth);
 int athmi_get_fusion(const athmi_grid *g, int *depth);
+/* Tile order of the corrector and final kernels: 0 = row by row, s > 0 =
+ * strips of s tile columns (64 s cells wide), each walked row by row, so that
+ * the tile dispatched just before a tile is mostly the one above it and the
+ * halo rows they share are still in the XCD's L2. Picked by the autotuner;
+ * results are identical. */
+int athmi_set_strip(athmi_grid *g, int tile_cols);
+int athmi_get_strip(const athmi_grid *g, int *tile_cols);
 /* The current launch shape (any pointer may be NULL) */
 int athmi_get_shape(const athmi_grid *g, int *x2_seg, int *persist, int *xcd_remap,
                     int *corr_rows);
 int athmi_use_graph(athmi_grid *g, int on);
 /* Autotuner: time every launch shape (x2 predictor form x x2 marching depth
  * 16..128 x persistent workgroups per CU 0..32 x tile order, then the
- * corrector's tile height 4/2/1 and the fusion depth at the best of those)
+ * corrector's tile height 4/2/1, the corrector and final kernels' strip
+ * width 0/2/4/8 and the fusion depth at the best of those)
  * over `steps` main-loop steps from the current state through the step
  * graph, restore the state and clocks; the four
  * fastest shapes then run three more trials each, in turn, and the lowest

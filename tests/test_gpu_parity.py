@@ -42,7 +42,7 @@ CASES = [
 
 
 # kernel forms: (x2 predictor form, fusion depth); results must not depend on them
-FORMS = [(0, 0), (1, 0), (1, 1)]
+FORMS = [(0, 0), (1, 0), (1, 1), (1, 2)]
 
 
 @pytest.mark.parametrize("forms", FORMS, ids=lambda f: f"x2form{f[0]}_fusion{f[1]}")
@@ -67,14 +67,15 @@ def test_exact_steps_match_oracle(problem, overrides, nsteps, flux, forms):
     gpu.close()
 
 
-@pytest.mark.parametrize("bcs", [(2, 2, 2, 2), (1, 1, 1, 1), (1, 2, 4, 4), (4, 4, 2, 1)])
-def test_exact_boundaries_match_oracle(bcs):
+@pytest.mark.parametrize("bcs,fusion", [((2, 2, 2, 2), 1), ((1, 1, 1, 1), 2), ((1, 2, 4, 4), 2),
+                                        ((4, 4, 2, 1), 1)])
+def test_exact_boundaries_match_oracle(bcs, fusion):
     pkg = load_pkg()
     ov = ["grid/Nx1=48", "grid/Nx2=40", f"grid/ibc_x1={bcs[0]}", f"grid/obc_x1={bcs[1]}",
           f"grid/ibc_x2={bcs[2]}", f"grid/obc_x2={bcs[3]}", "problem/radius=0.25"]
-    say(f"[gpu] exact blast boundaries {bcs}")
+    say(f"[gpu] exact blast boundaries {bcs} fusion {fusion}")
     gpu, par, tlim, _ = pkg.setup_problem("blast", ov, variant="exact")
-    gpu.set_fusion(1)
+    gpu.set_fusion(fusion)
     ora, _, _ = oracle_setup("blast", ov)
     compare_exact(gpu, ora, "initial ghosts")
     gpu.set_limits(tlim, 25)
@@ -83,6 +84,53 @@ def test_exact_boundaries_match_oracle(bcs):
         ora.step(tlim)
     compare_exact(gpu, ora, f"boundaries {bcs} after 25 steps")
     gpu.close()
+
+
+@pytest.mark.parametrize("rows", [1, 2, 4])
+@pytest.mark.parametrize("nx1,nx2", [(61, 61), (64, 62), (125, 29)])
+def test_fused_corner_tile_edges(nx1, nx2, rows):
+    """Fusion depth 2: the x1 corrector's tiles compute the Step 4 corners and
+    store those they own. Grids where the corner range's last row or column
+    (je+2, ie+2) falls just outside the last tile's own rows or columns
+    ((nx + 3) a multiple of the tile size) or inside them; both fluxes."""
+    pkg = load_pkg()
+    ov = [f"grid/Nx1={nx1}", f"grid/Nx2={nx2}", "problem/radius=0.2"]
+    for flux in ("roe", "hlle"):
+        say(f"[gpu] fusion 2 blast {nx1}x{nx2} corrector rows {rows} {flux}")
+        gpu, par, tlim, _ = pkg.setup_problem("blast", ov, flux=flux, variant="exact")
+        gpu.set_fusion(2)
+        gpu.set_corr_rows(rows)
+        ora, _, _ = oracle_setup("blast", ov, flux=flux)
+        gpu.set_limits(tlim, 12)
+        for s in range(12):
+            gpu.step()
+            ora.step(tlim)
+        compare_exact(gpu, ora, f"fusion 2 {nx1}x{nx2} rows {rows} {flux} after 12 steps")
+        gpu.close()
+
+
+@pytest.mark.parametrize("strip", [2, 3, 4, 8])
+def test_strip_tile_order(strip):
+    """Corrector and final tiles walked in strips of `strip` tile columns
+    (athmi_set_strip): five tile columns, so the strips are uneven (2+2+1,
+    3+2, 4+1) or wider than the grid (8: row order); every corrector tile
+    height and fusion depth 1 and 2."""
+    pkg = load_pkg()
+    ov = ["grid/Nx1=300", "grid/Nx2=72", "problem/radius=0.2"]
+    for rows, fusion in ((1, 1), (2, 2), (4, 1), (4, 2)):
+        say(f"[gpu] strip {strip} blast 300x72 corrector rows {rows} fusion {fusion}")
+        gpu, par, tlim, _ = pkg.setup_problem("blast", ov, variant="exact")
+        gpu.set_strip(strip)
+        gpu.set_corr_rows(rows)
+        gpu.set_fusion(fusion)
+        assert gpu.strip() == strip
+        ora, _, _ = oracle_setup("blast", ov)
+        gpu.set_limits(tlim, 10)
+        for s in range(10):
+            gpu.step()
+            ora.step(tlim)
+        compare_exact(gpu, ora, f"strip {strip} rows {rows} fusion {fusion} after 10 steps")
+        gpu.close()
 
 
 def test_upload_download_round_trip_and_restep():
