@@ -359,11 +359,16 @@ def main():
         g.step()
     g.sync()
     log(f"[bench] timing {args.steps} steps")
+    n0 = g.clock(0)[2]
     t0 = time.perf_counter()
     for _ in range(args.steps):
         g.step()
     g.sync()
     t1 = time.perf_counter()
+    # every timed call advanced the instance (a stopped instance's step does no work)
+    if g.clock(0)[2] - n0 != args.steps:
+        raise RuntimeError(f"the timed region advanced {g.clock(0)[2] - n0} of {args.steps} steps; "
+                           f"failure word {g.failure(0)}")
     sec = t1 - t0
     cells = n * n
     value = cells * args.steps / sec

@@ -164,11 +164,11 @@ int athmi_use_graph(athmi_grid *g, int on);
  * 16..128 x persistent workgroups per CU 0..32 x tile order, then the
  * corrector's tile height 4/2/1, the corrector and final kernels' strip
  * width 0/2/4/8 and the fusion depth at the best of those)
- * over `steps` main-loop steps from the current state through the step
- * graph, restore the state and clocks; the four
- * fastest shapes then run three more trials each, in turn, and the lowest
- * total wins (returned with its mean ms per step; athmi_get_shape gives all
- * of it). No reference counterpart (the reference launches fixed 32-thread
+ * by the better of two runs of `steps` main-loop steps from the current state
+ * through the step graph, restore the state and clocks; the six fastest
+ * shapes then run three more trials each, in turn, and the lowest total wins
+ * (returned with its mean ms per step; athmi_get_shape and the getters give
+ * all of it). No reference counterpart (the reference launches fixed 32-thread
  * blocks, integrate_2d_cuda.cu:93-193). */
 int athmi_autotune(athmi_grid *g, int steps, int *x2_seg, int *persist, int *xcd_remap,
                    double *ms_per_step);
