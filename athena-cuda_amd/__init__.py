@@ -78,6 +78,8 @@ SIGNATURES = {
     "athmi_get_fusion": (_I, [_P, _IP]),
     "athmi_set_strip": (_I, [_P, _I]),
     "athmi_get_strip": (_I, [_P, _IP]),
+    "athmi_set_final_rows": (_I, [_P, _I]),
+    "athmi_get_final_rows": (_I, [_P, _IP]),
     "athmi_get_shape": (_I, [_P, _IP, _IP, _IP, _IP]),
     "athmi_device_state": (_I, [_P, _I, ctypes.POINTER(_P), _IP]),
     "athmi_pgen_blast": (_I, [_P, _I, _D, _D, _D, _D, _D, _D, _D]),
@@ -331,6 +333,15 @@ class Grid:
     def strip(self):
         k = ctypes.c_int()
         _check(lib().athmi_get_strip(self._h, ctypes.byref(k)), "get_strip")
+        return k.value
+
+    def set_final_rows(self, rows):
+        """Rows of a final-update tile: 4, 8 or 16"""
+        _check(lib().athmi_set_final_rows(self._h, int(rows)), "set_final_rows")
+
+    def final_rows(self):
+        k = ctypes.c_int()
+        _check(lib().athmi_get_final_rows(self._h, ctypes.byref(k)), "get_final_rows")
         return k.value
 
     def set_tile_order(self, xcd_remap=1):

@@ -64,10 +64,11 @@ int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
   g->bl.shape.x2_seg = (int)std::max(4L, std::min(16L, cols / 2048));
   g->bl.shape.persist = 0;
   g->bl.shape.xcd_remap = 1;
-  g->bl.shape.corr_rows = kTileY;
+  g->bl.shape.corr_rows = 2;  // 64 x 2 corrector tiles (at 4096^2: 1.46 against 1.60 ms for 64 x 4)
   g->bl.shape.x2_kind = 1;  // the two-wave x2 predictor (1.79 against 2.05 ms at 4096^2)
   g->bl.shape.fuse_corner = 1;  // Step 9 corners inside k_final (the autotuner's pick at 4096^2)
   g->bl.shape.strip = 0;        // corrector and final tiles row by row
+  g->bl.shape.final_rows = 8;   // 64 x 8 final tiles (1.10 -> 1.07 ms against 64 x 4 at 4096^2)
   // device memory: per instance 2*NSTATE state planes + NSCRATCH scratch planes
   std::vector<size_t> off(n);
   size_t total = 0;
@@ -536,8 +537,7 @@ int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd
     }
     return r;
   };
-  // tile order x x2 marching depth x persistent occupancy, then the
-  // corrector's tile height at the best of those
+  // tile order x x2 marching depth x persistent occupancy
   for (int x2 = 0; !rc && x2 < 2; x2++)
     for (int c = 0; !rc && c < 2; c++)
       for (int a = 0; !rc && a < (int)(sizeof(kSeg) / sizeof(int)); a++)
@@ -549,50 +549,81 @@ int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd
           sh.xcd_remap = c;
           rc = sweep(sh);
         }
-  const LaunchShape base = best_shape;
-  for (int rows : {4, 2, 1}) {
-    if (rc || rows == base.corr_rows) continue;
-    LaunchShape sh = base;
-    sh.corr_rows = rows;
-    rc = sweep(sh);
-  }
+  // then one dimension at a time at the best shape so far. Each stage times
+  // its base again beside the variants: the base's earlier time is the lowest
+  // of many short noisy trials, biased low, and a variant measured once would
+  // rarely beat it even when faster
+  auto stage = [&](const std::vector<LaunchShape>& variants) -> int {
+    double stage_best = 0.0;
+    LaunchShape stage_shape = best_shape;
+    int r = trial(best_shape, &stage_best);
+    if (r) return r;
+    timed.emplace_back(stage_best, best_shape);
+    for (const LaunchShape& sh : variants) {
+      double ms = 0.0;
+      if ((r = trial(sh, &ms))) return r;
+      timed.emplace_back(ms, sh);
+      if (ms < stage_best) {
+        stage_best = ms;
+        stage_shape = sh;
+      }
+    }
+    best = stage_best;
+    best_shape = stage_shape;
+    return ATHMI_OK;
+  };
+  auto variants_of = [&](std::initializer_list<int> values, int LaunchShape::*field,
+                         int current) {
+    std::vector<LaunchShape> v;
+    for (int x : values)
+      if (x != current) {
+        LaunchShape sh = best_shape;
+        sh.*field = x;
+        v.push_back(sh);
+      }
+    return v;
+  };
+  // the corrector's tile height
+  if (!rc) rc = stage(variants_of({4, 2, 1}, &LaunchShape::corr_rows, best_shape.corr_rows));
   // tile order of the corrector and final kernels: whole rows or strips
-  const LaunchShape base1 = best_shape;
-  for (int strip : {0, 2, 4, 8}) {
-    if (rc || strip == base1.strip) continue;
-    LaunchShape sh = base1;
-    sh.strip = strip;
-    rc = sweep(sh);
-  }
+  if (!rc) rc = stage(variants_of({0, 2, 4, 8}, &LaunchShape::strip, best_shape.strip));
+  // height of the final-update tiles
+  if (!rc)
+    rc = stage(variants_of({4, 8, 16}, &LaunchShape::final_rows,
+                           final_rows_of(best_shape.final_rows)));
   // fusion depth: the corner emf of Steps 4 and 9 in kernels of their own
   // (0), the Step 9 corners inside the final kernel's tiles (1), and also the
   // Step 4 corners inside the x1 corrector's tiles (2)
-  const LaunchShape base2 = best_shape;
-  for (int depth = 0; !rc && depth <= 2; depth++) {
-    if (depth == base2.fuse_corner) continue;
-    LaunchShape sh = base2;
-    sh.fuse_corner = depth;
-    rc = sweep(sh);
-  }
+  if (!rc) rc = stage(variants_of({0, 1, 2}, &LaunchShape::fuse_corner, best_shape.fuse_corner));
   // single short trials differ by about 1% from run to run: the fastest few
-  // shapes run again, in turn, and the lowest total time wins
+  // shapes, and the stages' winner, run again in turn, and the lowest total
+  // time wins
   if (!rc && timed.size() > 1) {
     std::stable_sort(timed.begin(), timed.end(),
                      [](const std::pair<double, LaunchShape>& x,
                         const std::pair<double, LaunchShape>& y) { return x.first < y.first; });
-    const size_t nf = std::min<size_t>(kFinalists, timed.size());
+    auto same = [](const LaunchShape& x, const LaunchShape& y) {
+      return std::memcmp(&x, &y, sizeof(LaunchShape)) == 0;
+    };
+    std::vector<LaunchShape> fin{best_shape};
+    for (size_t k = 0; k < timed.size() && fin.size() < kFinalists; k++) {
+      bool dup = false;
+      for (const LaunchShape& f : fin) dup = dup || same(f, timed[k].second);
+      if (!dup) fin.push_back(timed[k].second);
+    }
+    const size_t nf = fin.size();
     std::vector<double> total(nf, 0.0);
     for (int round = 0; !rc && round < kFinalRounds; round++)
       for (size_t f = 0; !rc && f < nf; f++) {
         double ms = 0.0;
-        rc = trial(timed[f].second, &ms);
+        rc = trial(fin[f], &ms);
         total[f] += ms;
       }
     if (!rc) {
       size_t w = 0;
       for (size_t f = 1; f < nf; f++)
         if (total[f] < total[w]) w = f;
-      best_shape = timed[w].second;
+      best_shape = fin[w];
       best = total[w] / kFinalRounds;
     }
   }
@@ -635,6 +666,21 @@ int athmi_set_strip(athmi_grid* g, int tile_cols) {
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->bl.shape.strip = tile_cols;
   return g->refresh_tiles();
+}
+
+int athmi_set_final_rows(athmi_grid* g, int rows) {
+  if (!g) return fail(ATHMI_EINVAL, "null handle");
+  if (rows != 4 && rows != 8 && rows != 16)
+    return fail(ATHMI_EINVAL, "athmi_set_final_rows: rows must be 4, 8 or 16");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->bl.shape.final_rows = rows;
+  return g->refresh_tiles();
+}
+
+int athmi_get_final_rows(const athmi_grid* g, int* rows) {
+  if (!g || !rows) return fail(ATHMI_EINVAL, "athmi_get_final_rows: null argument");
+  *rows = final_rows_of(g->bl.shape.final_rows);
+  return ATHMI_OK;
 }
 
 int athmi_get_strip(const athmi_grid* g, int* tile_cols) {

@@ -94,7 +94,8 @@ struct athmi_grid {
       Inst& I = hinst[k];
       for (int kind = 0; kind < NKINDS; kind++) {
         I.tile0[kind] = tot[kind];
-        tot[kind] += tiles_of(kind, I.nx1, I.nx2, bl.shape.x2_seg, bl.shape.corr_rows);
+        tot[kind] += tiles_of(kind, I.nx1, I.nx2, bl.shape.x2_seg, bl.shape.corr_rows,
+                              bl.shape.final_rows);
       }
     }
     for (int kind = 0; kind < NKINDS; kind++) bl.ntiles[kind] = tot[kind];

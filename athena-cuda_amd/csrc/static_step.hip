@@ -171,11 +171,13 @@ int athmi_integrate_static(const athmi_desc* desc, double dt, const double* stat
   bl.shape.x2_seg = (int)(cols / 2048 < 4 ? 4 : (cols / 2048 > 16 ? 16 : cols / 2048));
   bl.shape.persist = 0;
   bl.shape.xcd_remap = 1;
-  bl.shape.corr_rows = kTileY;
+  bl.shape.corr_rows = 2;
+  bl.shape.final_rows = 8;
   bl.shape.x2_kind = 1;
   for (int kind = 0; kind < NKINDS; kind++) {
     I.tile0[kind] = 0;
-    bl.ntiles[kind] = tiles_of(kind, d.nx1, d.nx2, bl.shape.x2_seg, bl.shape.corr_rows);
+    bl.ntiles[kind] = tiles_of(kind, d.nx1, d.nx2, bl.shape.x2_seg, bl.shape.corr_rows,
+                              bl.shape.final_rows);
   }
   bl.flux_mask = 1 << d.flux;
 

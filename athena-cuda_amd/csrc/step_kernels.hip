@@ -143,8 +143,9 @@ __device__ __forceinline__ Cons7 get7(const double* base, size_t ps, uint32_t k)
                LD(base + 4 * ps, k), LD(base + 5 * ps, k), LD(base + 6 * ps, k)};
 }
 
-__device__ __forceinline__ double block_max256(double v, double* red) {
-  // wave-level max with cross-lane shuffles, then across the 4 waves via LDS
+__device__ __forceinline__ double block_max(double v, double* red) {
+  // wave-level max with cross-lane shuffles, then across the block's waves
+  // via LDS (red: one slot per wave)
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = dmax(v, __shfl_xor(v, off, 64));
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
@@ -610,7 +611,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_corner(const Inst* __restric
 // j0..j0+TY and columns i0..i0+kTileX into LDS from the predictor fluxes, and
 // stores the corners it owns -- together the range [js-1,je+2]x[is-1,ie+2] --
 // in the corner plane for the x2 part (no k_corner<pred> launch).
-constexpr int kCornerRows = kTileY + 1, kCornerCols = kTileX + 1;
+constexpr int kCornerCols = kTileX + 1;
 template <int PART, int TY, int FLUX, bool CP = false>  // TY: rows of a tile of kTileX x TY faces
 __global__ void __launch_bounds__(kTileX* TY, 2) k_corr(const Inst* __restrict__ insts,
                                                             int ninst, int ntiles, int remap,
@@ -827,16 +828,18 @@ __device__ __forceinline__ int cell_fail_bits(const Inst& g, double d, double M1
 // Steps 9, 11a, 11b, 13 (:473-534) over [js,je+1]x[is,ie+1] into the other
 // ping-pong buffer, and the new_dt maximum of the updated interior. The B2c
 // update of 11a and the B1c update of 11b are dropped: Step 13 overwrites them.
-// FUSED: the tile first computes the Step 9 corner emf it reads -- (kTileY+1)
+// FUSED: the tile first computes the Step 9 corner emf it reads -- (TY+1)
 // x (kTileX+1) corners -- into LDS from the corrector fluxes, instead of
 // reading the plane k_corner<false> wrote (one launch and the corner plane's
-// write and read fewer).
-template <bool FUSED>
-__global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict__ insts,
-                                                           int ninst, int ntiles, int remap,
-                                                           int strip) {
-  __shared__ double red[4];
-  __shared__ double cemf[FUSED ? kCornerRows * kCornerCols : 1];
+// write and read fewer). TY: rows of a tile of kTileX x TY cells; the taller,
+// the smaller the share of halo rows (corners and x2 fluxes of the row below)
+// that the tile reads a second time.
+template <bool FUSED, int TY>
+__global__ void __launch_bounds__(kTileX* TY) k_final(const Inst* __restrict__ insts,
+                                                       int ninst, int ntiles, int remap,
+                                                       int strip) {
+  __shared__ double red[TY];  // kTileX x TY threads: TY waves
+  __shared__ double cemf[FUSED ? (TY + 1) * kCornerCols : 1];
   for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
     const int t = xcd_tile(t0, ntiles, remap);
     const Inst& g = insts[find_inst(insts, ninst, K_FINAL, t)];
@@ -844,17 +847,17 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict
     if (!clk->active) continue;
     const int local = t - g.tile0[K_FINAL];
     int tx, ty;
-    tile_pos(local, dcdiv(g.nx1 + 1, kTileX), dcdiv(g.nx2 + 1, kTileY), strip, tx, ty);
-    const int i0 = g.is + tx * kTileX, j0 = g.js + ty * kTileY;
+    tile_pos(local, dcdiv(g.nx1 + 1, kTileX), dcdiv(g.nx2 + 1, TY), strip, tx, ty);
+    const int i0 = g.is + tx * kTileX, j0 = g.js + ty * TY;
     const int i = i0 + threadIdx.x;
     const int j = j0 + threadIdx.y;
     if (FUSED) {
-      // corners [js,je+1] x [is,ie+1] of this tile's rows j0..j0+kTileY and
+      // corners [js,je+1] x [is,ie+1] of this tile's rows j0..j0+TY and
       // columns i0..i0+kTileX (outside that range: never read)
       const double* sc = g.scratch;
       const size_t ps = g.ps;
       const int tid = threadIdx.y * kTileX + threadIdx.x;
-      for (int c = tid; c < kCornerRows * kCornerCols; c += kTileX * kTileY) {
+      for (int c = tid; c < (TY + 1) * kCornerCols; c += kTileX * TY) {
         const int cj = j0 + c / kCornerCols, ci = i0 + c % kCornerCols;
         double e = 0.0;
         if (cj <= g.je + 1 && ci <= g.ie + 1)
@@ -925,7 +928,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_final(const Inst* __restrict
         if (bits) atomicOr(&clk->fail, bits);
       }
     }
-    const double m = block_max256(dti, red);
+    const double m = block_max(dti, red);
     if (threadIdx.x == 0 && threadIdx.y == 0) atomic_max_pos(&clk->dti_slot[t % kDtSlots], m);
   }
 }
@@ -1001,7 +1004,7 @@ __global__ void __launch_bounds__(256) k_begin(const Inst* __restrict__ insts, i
 
 __device__ __forceinline__ double reduce_slots(const Clock* c, double* red) {
   double m = __longlong_as_double((long long)c->dti_slot[threadIdx.x]);
-  return block_max256(m, red);
+  return block_max(m, red);
 }
 
 // mode 0 (end of a main-loop step, main.c:289-292 + new_dt.c:93-97):
@@ -1046,7 +1049,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_dti(const Inst* __restrict__
       dti = inverse_dt(g, LD(st + F_D * ps, k), LD(st + F_M1 * ps, k), LD(st + F_M2 * ps, k),
                        LD(st + F_M3 * ps, k), LD(st + F_E * ps, k));
     }
-    const double m = block_max256(dti, red);
+    const double m = block_max(dti, red);
     if (threadIdx.x == 0 && threadIdx.y == 0) atomic_max_pos(&clk->dti_slot[t % kDtSlots], m);
   }
 }
@@ -1088,7 +1091,7 @@ __global__ void __launch_bounds__(kTileX* kTileY) k_divb(const Inst* __restrict_
       v = fabs((LD(st + F_B1I * ps, k + 1) - LD(st + F_B1I * ps, k)) / g.dx1 +
                (LD(st + F_B2I * ps, k + g.pitch) - LD(st + F_B2I * ps, k)) / g.dx2);
     }
-    const double m = block_max256(v, red);
+    const double m = block_max(v, red);
     if (threadIdx.x == 0 && threadIdx.y == 0) atomic_max_pos(&clk->divb_bits, m);
   }
 }
@@ -1114,6 +1117,7 @@ static PredX2Kernel pred_x2_kernel(bool save, int flux, int kind) {
   return save ? k_pred_x2<true, kFluxRoe> : k_pred_x2<false, kFluxRoe>;
 }
 typedef void (*CorrKernel)(const Inst*, int, int, int, int);
+typedef void (*FinalKernel)(const Inst*, int, int, int, int);
 // the corrector instantiation for a part (1: x1 faces, 2: x2 faces and Step
 // 7), tile height and Riemann solver
 template <int PART, bool CP>
@@ -1207,11 +1211,17 @@ hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
       hipLaunchKernelGGL(k_corner<false>, dim3(grid_of(b, K_CORNER_C, 8)), b2, 0, s, b.insts,
                          b.ninst, b.ntiles[K_CORNER_C], b.shape.xcd_remap);
       break;
-    case SK_FINAL:
-      hipLaunchKernelGGL(b.shape.fuse_corner ? k_final<true> : k_final<false>,
-                         dim3(grid_of(b, K_FINAL, 8)), b2, 0, s, b.insts, b.ninst,
-                         b.ntiles[K_FINAL], b.shape.xcd_remap, b.shape.strip);
+    case SK_FINAL: {
+      // 32 waves per CU whatever the tile height
+      const int ty = final_rows_of(b.shape.final_rows);
+      const bool f = b.shape.fuse_corner != 0;
+      const FinalKernel fn = ty == 16 ? (f ? k_final<true, 16> : k_final<false, 16>)
+                             : ty == 8 ? (f ? k_final<true, 8> : k_final<false, 8>)
+                                       : (f ? k_final<true, 4> : k_final<false, 4>);
+      hipLaunchKernelGGL(fn, dim3(grid_of(b, K_FINAL, 32 / ty)), dim3(kTileX, ty), 0, s, b.insts,
+                         b.ninst, b.ntiles[K_FINAL], b.shape.xcd_remap, b.shape.strip);
       break;
+    }
     case SK_GHOST_CUR:
     case SK_GHOST_NEXT:
     case SK_GHOST_CARRY: {

@@ -99,9 +99,13 @@ constexpr int kGhostBlock = 256;
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
+// rows of a final-update tile (64 x 4, 8 or 16 cells; anything else: 4)
+inline int final_rows_of(int rows) { return (rows == 8 || rows == 16) ? rows : kTileY; }
+
 // number of tiles of kind k for a grid of nx1 x nx2 zones (x2seg: faces
-// marched per thread by the x2 predictor)
-inline int tiles_of(int k, int nx1, int nx2, int x2seg, int corr_rows) {
+// marched per thread by the x2 predictor; corr_rows, final_rows: tile heights
+// of the corrector and the final update)
+inline int tiles_of(int k, int nx1, int nx2, int x2seg, int corr_rows, int final_rows) {
   const int sx = nx1 + 8;
   switch (k) {
     case K_PRED_X1: return (nx2 + 4) * cdiv(nx1 + 4, kX1Faces);
@@ -109,7 +113,7 @@ inline int tiles_of(int k, int nx1, int nx2, int x2seg, int corr_rows) {
     case K_CORR: return cdiv(nx1 + 3, kTileX) * cdiv(nx2 + 3, corr_rows);
     case K_CORNER_P: return cdiv(nx1 + 4, kTileX) * cdiv(nx2 + 4, kTileY);
     case K_CORNER_C: return cdiv(nx1 + 2, kTileX) * cdiv(nx2 + 2, kTileY);
-    case K_FINAL: return cdiv(nx1 + 1, kTileX) * cdiv(nx2 + 1, kTileY);
+    case K_FINAL: return cdiv(nx1 + 1, kTileX) * cdiv(nx2 + 1, final_rows_of(final_rows));
     case K_GHOST: return cdiv(8 * sx + (nx2 + 1) * 8, kGhostBlock);
     default: return cdiv(nx1, kTileX) * cdiv(nx2, kTileY);
   }
@@ -129,6 +133,7 @@ struct LaunchShape {
                      // corrector's tiles (no k_corner<pred> launch)
   int strip;         // tile order of the corrector and final kernels: 0 = row by row,
                      // s > 0 = strips of s tile columns walked row by row
+  int final_rows;    // rows of a final-update tile: 4, 8 or 16 (final_rows_of)
 };
 
 // Everything a launcher needs for one batch (device descriptor array, tile

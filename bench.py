@@ -150,9 +150,11 @@ def batched_leg(pkg, args, plain_rate):
     shape = None
     if not args.no_tune:
         shape = gb.autotune(2)
-        shape = gb.launch_shape() + (shape[3], gb.x2_kind(), gb.fusion(), gb.strip())
+        shape = gb.launch_shape() + (shape[3], gb.x2_kind(), gb.fusion(), gb.strip(),
+                                     gb.final_rows())
         log(f"[bench] batch shape x2_seg={shape[0]} persist={shape[1]} xcd_remap={shape[2]} "
             f"corr_rows={shape[3]} x2_form={shape[5]} fusion={shape[6]} strip={shape[7]} "
+            f"final_rows={shape[8]} "
             f"({shape[4]:.3f} ms/step)")
     for _ in range(args.batch_warmup):
         gb.step()
@@ -178,7 +180,7 @@ def batched_leg(pkg, args, plain_rate):
             "shape": None if shape is None else {"x2_seg": shape[0], "persist": shape[1],
                                                  "xcd_remap": shape[2], "corr_rows": shape[3],
                                                  "x2_form": shape[5], "fusion": shape[6],
-                                                 "strip": shape[7]}}
+                                                 "strip": shape[7], "final_rows": shape[8]}}
 
 
 def adjoint_leg(pkg, args):
@@ -299,7 +301,8 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=3)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--shape", default=None,
-                    help="x2_seg,persist,xcd_remap,corr_rows,x2_form,fusion[,strip]: use this "
+                    help="x2_seg,persist,xcd_remap,corr_rows,x2_form,fusion[,strip[,final_rows]]: "
+                         "use this "
                          "launch "
                          "shape for the plain step instead of autotuning (profiling runs)")
     ap.add_argument("--adjoint", type=int, default=1, help="0: skip the adjoint leg")
@@ -347,13 +350,16 @@ def main():
         g.set_x2_kind(v[4])
         g.set_fusion(v[5])
         g.set_strip(v[6] if len(v) > 6 else 0)
-        shape = tuple(v[:4]) + (float("nan"), v[4], v[5], v[6] if len(v) > 6 else 0)
+        g.set_final_rows(v[7] if len(v) > 7 else 8)
+        shape = tuple(v[:4]) + (float("nan"), v[4], v[5], v[6] if len(v) > 6 else 0,
+                                 v[7] if len(v) > 7 else 8)
         log(f"[bench] fixed shape {args.shape}")
     elif not args.no_tune:
         shape = g.autotune(3)
-        shape = g.launch_shape() + (shape[3], g.x2_kind(), g.fusion(), g.strip())
+        shape = g.launch_shape() + (shape[3], g.x2_kind(), g.fusion(), g.strip(), g.final_rows())
         log(f"[bench] autotuned shape x2_seg={shape[0]} persist={shape[1]} xcd_remap={shape[2]} "
             f"corr_rows={shape[3]} x2_form={shape[5]} fusion={shape[6]} strip={shape[7]} "
+            f"final_rows={shape[8]} "
             f"({shape[4]:.3f} ms/step)")
     for _ in range(args.warmup):
         g.step()
@@ -423,7 +429,7 @@ def main():
                    "shape": None if shape is None else {"x2_seg": shape[0], "persist": shape[1],
                                                  "xcd_remap": shape[2], "corr_rows": shape[3],
                                                  "x2_form": shape[5], "fusion": shape[6],
-                                                 "strip": shape[7]}},
+                                                 "strip": shape[7], "final_rows": shape[8]}},
         "roofline": roofline,
         "divb": {"max_abs": divb, "max_abs_times_dx": divb * dx,
                  "steps": args.warmup + args.steps},

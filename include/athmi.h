@@ -133,7 +133,7 @@ int athmi_set_shape(athmi_grid *g, int x2_seg, int persist);
 /* Tile order: 1 = each of the 8 XCDs gets one contiguous run of tiles (its L2
  * serves the neighbour rows), 0 = launch order. Results are identical. */
 int athmi_set_tile_order(athmi_grid *g, int xcd_remap);
-/* Rows of a corrector tile (64 x rows faces: 1, 2 or 4; default 4). Results
+/* Rows of a corrector tile (64 x rows faces: 1, 2 or 4; default 2). Results
  * are identical. */
 int athmi_set_corr_rows(athmi_grid *g, int rows);
 /* Form of the x2 predictor: 0 = each lane marches its column with the PLM
@@ -156,6 +156,11 @@ int athmi_get_fusion(const athmi_grid *g, int *depth);
  * results are identical. */
 int athmi_set_strip(athmi_grid *g, int tile_cols);
 int athmi_get_strip(const athmi_grid *g, int *tile_cols);
+/* Rows of a final-update tile (64 x rows cells: 4, 8 or 16; default 8): the
+ * taller the tile, the smaller the share of its halo rows read twice.
+ * Picked by the autotuner; results are identical. */
+int athmi_set_final_rows(athmi_grid *g, int rows);
+int athmi_get_final_rows(const athmi_grid *g, int *rows);
 /* The current launch shape (any pointer may be NULL) */
 int athmi_get_shape(const athmi_grid *g, int *x2_seg, int *persist, int *xcd_remap,
                     int *corr_rows);
@@ -163,7 +168,8 @@ int athmi_use_graph(athmi_grid *g, int on);
 /* Autotuner: time every launch shape (x2 predictor form x x2 marching depth
  * 16..128 x persistent workgroups per CU 0..32 x tile order, then the
  * corrector's tile height 4/2/1, the corrector and final kernels' strip
- * width 0/2/4/8 and the fusion depth at the best of those)
+ * width 0/2/4/8, the final tile height 4/8/16 and the fusion depth at the
+ * best of those)
  * by the better of two runs of `steps` main-loop steps from the current state
  * through the step graph, restore the state and clocks; the six fastest
  * shapes then run three more trials each, in turn, and the lowest total wins

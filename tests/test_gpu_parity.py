@@ -133,6 +133,29 @@ def test_strip_tile_order(strip):
         gpu.close()
 
 
+@pytest.mark.parametrize("rows", [4, 8, 16])
+def test_final_tile_rows(rows):
+    """Final-update tiles of 64 x 4, 8 or 16 cells (athmi_set_final_rows):
+    73 rows of faces, so the last tile row is partial at every height; row by
+    row and in uneven strips, fusion depth 0 and 1."""
+    pkg = load_pkg()
+    ov = ["grid/Nx1=300", "grid/Nx2=72", "problem/radius=0.2"]
+    for strip, fusion in ((0, 1), (3, 1), (0, 0)):
+        say(f"[gpu] final rows {rows} blast 300x72 strip {strip} fusion {fusion}")
+        gpu, par, tlim, _ = pkg.setup_problem("blast", ov, variant="exact")
+        gpu.set_final_rows(rows)
+        gpu.set_strip(strip)
+        gpu.set_fusion(fusion)
+        assert gpu.final_rows() == rows
+        ora, _, _ = oracle_setup("blast", ov)
+        gpu.set_limits(tlim, 10)
+        for s in range(10):
+            gpu.step()
+            ora.step(tlim)
+        compare_exact(gpu, ora, f"final rows {rows} strip {strip} fusion {fusion} after 10 steps")
+        gpu.close()
+
+
 def test_upload_download_round_trip_and_restep():
     """athmi_upload / athmi_download stage the caller's arrays in the idle
     ping-pong buffer and convert on the device: a random padded state comes
