@@ -111,7 +111,7 @@ def test_adjoint_gradient_vs_finite_differences_where_no_branch_flips():
     direction and eps the oracle's branch fingerprints at x + eps v and
     x - eps v tell whether the difference quotient straddles a kink. Where
     nothing flips, FD must agree with the adjoint to 1e-6 relative (the
-    north-star bar); directions that flip are reported."""
+    path's bar for its gradients); directions that flip are reported."""
     import torch
     A = adj_mod()
     O = oracle_lib()

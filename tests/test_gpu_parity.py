@@ -4,8 +4,8 @@ itself bit-exact against the reference's CPU sources, see test_oracle_ref.py).
 The exact variant (-ffp-contract=off kernels) must agree with the oracle value
 for value (np.array_equal; -0.0 == 0.0) on every field, ghosts included, and on
 dt/time/nstep. The FMA variant must agree to L1-relative <= 1e-12 after 100
-steps on every conserved and face-B field, the tolerance of the north-star
-parity bar.
+steps on every conserved and face-B field, the floating-point parity
+tolerance of the path.
 """
 import numpy as np
 import pytest
