@@ -100,7 +100,11 @@ struct athmi_grid {
     }
     for (int kind = 0; kind < NKINDS; kind++) bl.ntiles[kind] = tot[kind];
     bl.flux_mask = 0;
-    for (int k = 0; k < n; k++) bl.flux_mask |= 1 << hinst[k].flux;
+    bl.stale_ghosts = 0;
+    for (int k = 0; k < n; k++) {
+      bl.flux_mask |= 1 << hinst[k].flux;
+      if (ghost_fill_reads_stale(hinst[k])) bl.stale_ghosts = 1;
+    }
     HIP_TRY(hipMemcpy(dinst, hinst.data(), sizeof(Inst) * n, hipMemcpyHostToDevice));
     drop_graph();
     return ATHMI_OK;

@@ -961,17 +961,22 @@ __global__ void __launch_bounds__(kGhostBlock) k_ghost(const Inst* __restrict__ 
     }
     double* buf = g.state[which ? (clk->cur ^ 1) : clk->cur];
     const double* src = (which == 2) ? g.state[clk->cur] : buf;
+    // where a stale source (a ghost's value before this fill, grids thinner
+    // than the ghost layers) is read: filling the other buffer, the current
+    // one holds the last fill's ghosts; filling in place, the launcher has
+    // carried them into the other buffer first
+    const double* pre = g.state[which == 1 ? clk->cur : (clk->cur ^ 1)];
     const size_t ps = g.ps;
     const uint32_t k = (uint32_t)(j * g.pitch + i);
 #pragma unroll
     for (int f = 0; f < NSTATE; f++) {
       int sj, si;
-      bool neg;
-      if (ghost_src(g, f, j, i, sj, si, neg)) {
+      bool neg, stale;
+      if (ghost_src(g, f, j, i, sj, si, neg, stale)) {
         if (which == 2) {
           ST(buf + f * ps, k, LD(src + f * ps, k));
         } else {
-          const double v = LD(src + f * ps, (size_t)sj * g.pitch + si);
+          const double v = LD((stale ? pre : src) + f * ps, (size_t)sj * g.pitch + si);
           ST(buf + f * ps, k, neg ? -v : v);
         }
       }
@@ -1226,6 +1231,11 @@ hipError_t ATHMI_FN(launch)(const void* bl, int kernel, hipStream_t s) {
     case SK_GHOST_NEXT:
     case SK_GHOST_CARRY: {
       const int which = kernel == SK_GHOST_CUR ? 0 : (kernel == SK_GHOST_NEXT ? 1 : 2);
+      // an in-place fill that reads stale ghosts reads them from the other
+      // buffer: carry them there first
+      if (which == 0 && b.stale_ghosts)
+        hipLaunchKernelGGL(k_ghost, dim3(grid_of(b, K_GHOST, 8)), dim3(kGhostBlock), 0, s,
+                           b.insts, b.ninst, b.ntiles[K_GHOST], 2, 0, b.shape.xcd_remap);
       hipLaunchKernelGGL(k_ghost, dim3(grid_of(b, K_GHOST, 8)), dim3(kGhostBlock), 0, s, b.insts,
                          b.ninst, b.ntiles[K_GHOST], which, kernel == SK_GHOST_NEXT ? 1 : 0,
                          b.shape.xcd_remap);

@@ -119,6 +119,13 @@ inline int tiles_of(int k, int nx1, int nx2, int x2seg, int corr_rows, int final
   }
 }
 
+// whether an instance's set_bvals reads ghosts as they were before it (an
+// inner reflecting fill across fewer cells than the four ghost layers copies
+// outer ghosts the outer fill has not reached yet, set_bvals.c:312-366)
+inline bool ghost_fill_reads_stale(const Inst& g) {
+  return (g.bc[0] == BC_REFLECT && g.nx1 < 4) || (g.bc[2] == BC_REFLECT && g.nx2 < 4);
+}
+
 // Launch shape chosen by the autotuner
 struct LaunchShape {
   int x2_seg;        // faces marched per thread in the x2 predictor
@@ -142,6 +149,7 @@ struct BatchLaunch {
   const Inst* insts;
   int ninst;
   int ntiles[NKINDS];
+  int stale_ghosts;  // some instance's ghost fill reads stale ghosts (ghost_map.hpp)
   LaunchShape shape;
   int ncu;
   int flux_mask;  // bit f set: some instance uses Riemann solver f (kFluxRoe, kFluxHlle)

@@ -156,6 +156,33 @@ def test_final_tile_rows(rows):
         gpu.close()
 
 
+@pytest.mark.parametrize("bcs", [(4, 4, 4, 4), (2, 2, 2, 2), (1, 1, 1, 1), (1, 2, 4, 4)])
+@pytest.mark.parametrize("nx1,nx2", [(2, 2), (3, 5), (4, 4), (2, 64), (64, 2), (4, 40), (40, 4),
+                                     (37, 53)])
+def test_exact_tiny_and_thin_grids_match_oracle(nx1, nx2, bcs):
+    """Grids thinner than the four ghost layers (the reference's ghost fill
+    then copies ghost cells it wrote earlier in the same pass), thin strips
+    and odd sizes, every boundary kind, both fluxes, stepped as the main loop
+    does until tlim or 12 steps."""
+    pkg = load_pkg()
+    ov = [f"grid/Nx1={nx1}", f"grid/Nx2={nx2}", "problem/radius=0.2",
+          f"grid/ibc_x1={bcs[0]}", f"grid/obc_x1={bcs[1]}", f"grid/ibc_x2={bcs[2]}",
+          f"grid/obc_x2={bcs[3]}"]
+    for flux in ("roe", "hlle"):
+        say(f"[gpu] tiny/thin blast {nx1}x{nx2} bcs {bcs} {flux}")
+        gpu, par, tlim, _ = pkg.setup_problem("blast", ov, flux=flux, variant="exact")
+        ora, _, _ = oracle_setup("blast", ov, flux=flux)
+        compare_exact(gpu, ora, f"{nx1}x{nx2} {bcs} {flux} initial state")
+        gpu.set_limits(tlim, 12)
+        for s in range(12):
+            if ora.clock()[1] >= tlim:
+                break
+            gpu.step()
+            ora.step(tlim)
+            compare_exact(gpu, ora, f"{nx1}x{nx2} {bcs} {flux} after {s + 1} steps")
+        gpu.close()
+
+
 def test_upload_download_round_trip_and_restep():
     """athmi_upload / athmi_download stage the caller's arrays in the idle
     ping-pong buffer and convert on the device: a random padded state comes
@@ -183,12 +210,16 @@ def test_upload_download_round_trip_and_restep():
     gpu.close()
 
 
-def test_exact_routines_one_by_one():
+@pytest.mark.parametrize("ov", [["grid/Nx1=40", "grid/Nx2=56"],
+                                ["grid/Nx1=3", "grid/Nx2=2", "problem/radius=0.2"] +
+                                [f"grid/{b}=1" for b in ("ibc_x1", "obc_x1", "ibc_x2", "obc_x2")]],
+                         ids=["blast40x56", "blast3x2_reflect"])
+def test_exact_routines_one_by_one(ov):
     """integrate_2d, new_dt and set_bvals as separate calls (the drop-in
-    granularity of the reference's main loop)."""
+    granularity of the reference's main loop); on a reflecting 3 x 2 grid the
+    in-place set_bvals reads outer ghosts as the previous fill left them."""
     pkg = load_pkg()
-    ov = ["grid/Nx1=40", "grid/Nx2=56"]
-    say("[gpu] exact routine-by-routine blast 40x56")
+    say(f"[gpu] exact routine-by-routine blast {ov}")
     gpu, par, tlim, _ = pkg.setup_problem("blast", ov, variant="exact")
     ora, _, _ = oracle_setup("blast", ov)
     for s in range(5):
