@@ -7,10 +7,12 @@ dt/time/nstep. The FMA variant must agree to L1-relative <= 1e-12 after 100
 steps on every conserved and face-B field, the floating-point parity
 tolerance of the path.
 """
+import os
+
 import numpy as np
 import pytest
 
-from helpers import interior, l1_rel, load_pkg, oracle_setup
+from helpers import INPUTS, interior, l1_rel, load_pkg, oracle_setup
 
 pytestmark = pytest.mark.gpu
 
@@ -181,6 +183,97 @@ def test_exact_tiny_and_thin_grids_match_oracle(nx1, nx2, bcs):
             ora.step(tlim)
             compare_exact(gpu, ora, f"{nx1}x{nx2} {bcs} {flux} after {s + 1} steps")
         gpu.close()
+
+
+@pytest.mark.parametrize("extra", [
+    ["problem/gamma=1.4"], ["problem/gamma=2.0"], ["problem/gamma=1.1"],
+    ["problem/b0=0.0"], ["problem/angle=0"], ["problem/angle=90"],
+    ["problem/b0=0.0", "problem/prat=1.0"], ["problem/pamb=1e-6"]],
+    ids=["gamma1.4", "gamma2", "gamma1.1", "nofield", "field_x1", "field_x2", "uniform",
+         "pamb1e-6"])
+def test_exact_parameter_edges_match_oracle(extra):
+    """Blast 40x56 away from its default parameters: other adiabatic indices
+    (gamma = 2 zeroes Gamma - 2 in the Roe eigensystem), no magnetic field
+    (the eigensystems' bt = 0 and degenerate-speed branches), the field along
+    x1 or x2 (bt = 0 in one sweep), a uniform state, and a near-vacuum ambient
+    (the pressure floor). Both fluxes, stepped as the main loop does until
+    tlim or 12 steps."""
+    pkg = load_pkg()
+    ov = ["grid/Nx1=40", "grid/Nx2=56"] + extra
+    for flux in ("roe", "hlle"):
+        say(f"[gpu] parameter edge {extra} {flux}")
+        gpu, par, tlim, _ = pkg.setup_problem("blast", ov, flux=flux, variant="exact")
+        ora, _, _ = oracle_setup("blast", ov, flux=flux)
+        gpu.set_limits(tlim, 12)
+        for s in range(12):
+            if ora.clock()[1] >= tlim:
+                break
+            gpu.step()
+            ora.step(tlim)
+            compare_exact(gpu, ora, f"{extra} {flux} after {s + 1} steps")
+        gpu.close()
+
+
+@pytest.mark.parametrize("problem,extra", [
+    ("field_loop", ["problem/vflow=0.0"]),
+    ("field_loop", ["problem/vflow=0.0", "problem/amp=0.0"]),
+    ("cpaw2d", ["grid/Nx1=32", "grid/Nx2=32", "problem/v_par=0.0"])],
+    ids=["static_loop", "static_uniform", "cpaw2d_standing"])
+def test_exact_zero_flow_matches_oracle(problem, extra):
+    """Zero mass flux through every face at the start: a static field loop
+    (the corner emf's third, averaging, upwind case everywhere, and the
+    characteristic tracing's ev == 0), a static uniform state, and the
+    standing circularly polarised Alfven wave."""
+    pkg = load_pkg()
+    for flux in ("roe", "hlle"):
+        say(f"[gpu] zero flow {problem} {extra} {flux}")
+        gpu, par, tlim, _ = pkg.setup_problem(problem, extra, flux=flux, variant="exact")
+        ora, _, _ = oracle_setup(problem, extra, flux=flux)
+        gpu.set_limits(tlim, 12)
+        for s in range(12):
+            gpu.step()
+            ora.step(tlim)
+            compare_exact(gpu, ora, f"{problem} {extra} {flux} after {s + 1} steps")
+        gpu.close()
+
+
+def test_batch_with_thin_grids_matches_oracle():
+    """One batch of blasts of 2x2 (reflecting), 64x40 (periodic), 3x5 (mixed)
+    and 40x4 (outflow, HLLE): the batch's in-place ghost fill carries every
+    instance's ghosts when one of them reads stale ones, and each instance
+    runs to its own tlim. Every instance equals its own oracle run."""
+    pkg = load_pkg()
+    cases = [(2, 2, (1, 1, 1, 1), "roe"), (64, 40, (4, 4, 4, 4), "roe"),
+             (3, 5, (1, 2, 4, 1), "roe"), (40, 4, (2, 2, 2, 2), "hlle")]
+    descs, params, oras = [], [], []
+    for nx1, nx2, bcs, flux in cases:
+        ov = [f"grid/Nx1={nx1}", f"grid/Nx2={nx2}", "problem/radius=0.2",
+              f"grid/ibc_x1={bcs[0]}", f"grid/obc_x1={bcs[1]}", f"grid/ibc_x2={bcs[2]}",
+              f"grid/obc_x2={bcs[3]}"]
+        par = pkg.read_athinput(os.path.join(INPUTS, "athinput.blast"), ov)
+        descs.append(pkg.desc_from_athinput(par, flux=flux))
+        params.append(par)
+        oras.append(oracle_setup("blast", ov, flux=flux))
+    say(f"[gpu] batch of {len(cases)} blasts with thin grids")
+    g = pkg.Grid(descs, variant="exact")
+    for k, par in enumerate(params):
+        pr = par["problem"]
+        g.pgen_blast(k, float(pr["radius"]), float(pr["pamb"]), float(pr["prat"]),
+                     float(pr["b0"]), float(pr["angle"]))
+    g.set_bvals()
+    g.new_dt()
+    tlim = oras[0][1]
+    g.set_limits(tlim, 12)
+    for s in range(12):
+        g.step()
+        for k, (ora, _, _) in enumerate(oras):
+            if ora.clock()[1] < tlim:
+                ora.step(tlim)
+    for k, (ora, _, _) in enumerate(oras):
+        for name, x, y in zip(FIELDS, g.download(k), ora.get()):
+            assert np.array_equal(x, y), f"instance {k} {cases[k]}: {name}"
+        assert g.clock(k) == ora.clock(), f"instance {k} clock"
+    g.close()
 
 
 def test_upload_download_round_trip_and_restep():
