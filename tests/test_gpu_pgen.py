@@ -35,7 +35,7 @@ HOST = {
     "field_loop": lambda g, k, p: g.pgen_field_loop(k, *p[:3], int(p[3])),
     "cpaw2d": lambda g, k, p: g.pgen_cpaw2d(k, *p),
 }
-SIZES = [(64, 48), (100, 64), (37, 90), (128, 128), (64, 200)]
+SIZES = [(64, 48), (100, 64), (37, 90), (128, 128), (64, 200), (2, 2), (3, 5)]
 
 
 def _pair(problem):
