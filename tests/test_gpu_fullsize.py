@@ -53,3 +53,41 @@ def test_bench_configuration_conserves_and_keeps_div_b():
     t1 = totals()
     assert np.all(np.abs(t1 - t0) <= 1e-12 * np.abs(t0)), (t0, t1)
     g.close()
+
+
+def test_largest_blast_steps_conserve_and_keep_div_b():
+    """The blast at 16384 x 16384: its 81 planes of 16392 x 16400 doubles
+    (state, the other ping-pong buffer and the 59 scratch planes) take 174 GB
+    of the GPU's 288 GB, and plane offsets reach 2.7e8 of the kernels' 32-bit
+    range. Three main-loop steps from the device generator: mass and energy
+    conserved to roundoff on the periodic box, div B at roundoff, the clock
+    advanced and no failure bit."""
+    pkg = load_pkg()
+    n = 16384
+    print(f"[fullsize] blast {n}x{n}: 3 steps, conservation and div B", flush=True)
+    desc = dict(nx1=n, nx2=n, x1min=-0.5, x1max=0.5, x2min=-0.75, x2max=0.75,
+                gamma=5.0 / 3.0, cour_no=0.4, bc=(4, 4, 4, 4), flux="roe")
+    g = pkg.Grid(desc)
+    g.pgen_device(0, [("blast", (0.1, 0.1, 100.0, 1.0, 45.0, 0.0, 0.0))])
+    g.set_bvals()
+    g.new_dt()
+    g.set_limits(1e30, -1)
+    dx = 1.0 / n
+
+    def totals():
+        gas = g.download(0)[0][4:4 + n, 4:4 + n]
+        return np.array([np.sum(gas[..., 0]), np.sum(gas[..., 4])])
+
+    t0 = totals()
+    for _ in range(3):
+        g.step()
+    t1 = totals()
+    dt, time, nstep = g.clock(0)
+    print(f"[fullsize] {n}^2: nstep {nstep}, time {time:.3e}, mass/energy {t0} -> {t1}",
+          flush=True)
+    assert nstep == 3 and np.isfinite(dt) and dt > 0.0
+    assert g.failure(0) == (0, -1)
+    assert np.all(np.abs(t1 - t0) <= 1e-12 * np.abs(t0)), (t0, t1)
+    # B is O(1): the roundoff of a few operations per step over dx
+    assert g.divb_max(0) * dx <= 3 * 4 * 2.0 ** -52
+    g.close()
