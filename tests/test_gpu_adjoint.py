@@ -26,6 +26,8 @@ CASES = [
     ("hlle_outflow", 24, 32, "hlle", (2, 2, 4, 4), 5, 2, False),
     ("roe_reflect_blast", 28, 28, "roe", (1, 1, 2, 1), 5, 3, True),
     ("hlle_blast_periodic", 24, 24, "hlle", (4, 4, 4, 4), 4, 4, True),
+    # the smallest grid the adjoint takes (its ghost sources are interior)
+    ("roe_smallest_mixed", 4, 5, "roe", (1, 2, 4, 4), 4, 2, False),
 ]
 
 
