@@ -163,15 +163,20 @@ def ref_grid(host, d, clock):
     return G
 
 
-def test_dropin_resident_main_loop_matches_oracle():
+@pytest.mark.parametrize("overrides", [
+    ["grid/Nx1=64", "grid/Nx2=48"],
+    ["grid/Nx1=5", "grid/Nx2=3", "problem/radius=0.2", "grid/ibc_x1=1", "grid/obc_x1=1",
+     "grid/ibc_x2=1", "grid/obc_x2=1"]], ids=["blast64x48", "blast5x3_reflect"])
+def test_dropin_resident_main_loop_matches_oracle(overrides):
     """Resident mode: the Grid crosses PCIe once; the host arrays are
     refreshed on request (athmi_dropin_sync_host) and equal the oracle's state
     there; a host-side change announced by athmi_dropin_mark_dirty is
-    uploaded by the next call."""
+    uploaded by the next call. On a reflecting grid thinner than the ghost
+    layers the device's set_bvals reads the ghosts its previous call left."""
     pkg = load_pkg()
     L = pkg.lib()
     bind_dropin(L)
-    problem, overrides, flux = "blast", ["grid/Nx1=64", "grid/Nx2=48"], "roe"
+    problem, flux = "blast", "roe"
     print(f"[gpu] drop-in resident {problem} {overrides}", flush=True)
     par = pkg.read_athinput(f"{INPUTS}/athinput.{problem}", overrides)
     d = pkg.desc_from_athinput(par, flux=flux)
@@ -206,7 +211,7 @@ def test_dropin_resident_main_loop_matches_oracle():
             if s == 4:
                 # a host-side edit of the state: both sides take it
                 gas, b1, b2, b3 = host.get()
-                gas[10:20, 10:20, 4] *= 1.5
+                gas[4:8, 4:6, 4] *= 1.5
                 host.set((gas, b1, b2, b3))
                 ora.put(gas, b1, b2, b3)
                 L.athmi_dropin_mark_dirty()
