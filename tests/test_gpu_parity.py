@@ -276,6 +276,48 @@ def test_batch_with_thin_grids_matches_oracle():
     g.close()
 
 
+def _random_configs(seed, n):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nx1 = int(rng.choice([2, 3, 4, 5, 7, 16, 33, 64, 96]))
+        nx2 = int(rng.choice([2, 3, 4, 6, 9, 17, 40, 65, 96]))
+        bcs = [int(b) for b in rng.choice([1, 2, 4], size=4)]
+        ov = [f"grid/Nx1={nx1}", f"grid/Nx2={nx2}",
+              f"grid/ibc_x1={bcs[0]}", f"grid/obc_x1={bcs[1]}", f"grid/ibc_x2={bcs[2]}",
+              f"grid/obc_x2={bcs[3]}",
+              f"problem/radius={rng.uniform(0.05, 0.3):.3f}",
+              f"problem/prat={rng.choice([1.0, 3.0, 10.0, 100.0])}",
+              f"problem/b0={rng.choice([0.0, 0.3, 1.0, 3.0])}",
+              f"problem/angle={rng.choice([0.0, 30.0, 45.0, 90.0])}",
+              f"problem/gamma={rng.choice([1.4, 5.0 / 3.0, 2.0])}"]
+        out.append((ov, str(rng.choice(["roe", "hlle"]))))
+    return out
+
+
+@pytest.mark.parametrize("k", range(4))
+def test_exact_random_configurations_match_oracle(k):
+    """A seeded sweep of ordinary blast configurations: sizes from 2 to 96 per
+    side, every combination of boundary kinds, either flux, and random
+    radius, pressure ratio, field strength and angle, adiabatic index; each
+    stepped as the main loop does (until tlim or 8 steps) and compared with
+    the oracle value for value after every step."""
+    pkg = load_pkg()
+    for ov, flux in _random_configs(1000 + k, 12):
+        say(f"[gpu] random config {flux} {ov}")
+        gpu, par, tlim, _ = pkg.setup_problem("blast", ov, flux=flux, variant="exact")
+        ora, _, _ = oracle_setup("blast", ov, flux=flux)
+        compare_exact(gpu, ora, f"{flux} {ov} initial state")
+        gpu.set_limits(tlim, 8)
+        for s in range(8):
+            if ora.clock()[1] >= tlim:
+                break
+            gpu.step()
+            ora.step(tlim)
+            compare_exact(gpu, ora, f"{flux} {ov} after {s + 1} steps")
+        gpu.close()
+
+
 def test_upload_download_round_trip_and_restep():
     """athmi_upload / athmi_download stage the caller's arrays in the idle
     ping-pong buffer and convert on the device: a random padded state comes
