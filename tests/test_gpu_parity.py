@@ -318,6 +318,52 @@ def test_exact_random_configurations_match_oracle(k):
         gpu.close()
 
 
+def _random_smooth_configs(seed, n):
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(n):
+        nx1 = int(rng.choice([2, 3, 5, 8, 24, 48, 80]))
+        nx2 = int(rng.choice([2, 4, 7, 12, 32, 64]))
+        bcs = [int(b) for b in rng.choice([1, 2, 4], size=4)]
+        ov = [f"grid/Nx1={nx1}", f"grid/Nx2={nx2}",
+              f"grid/ibc_x1={bcs[0]}", f"grid/obc_x1={bcs[1]}", f"grid/ibc_x2={bcs[2]}",
+              f"grid/obc_x2={bcs[3]}"]
+        if m % 2 == 0:
+            problem = "field_loop"
+            ov += [f"problem/rad={rng.uniform(0.1, 0.45):.3f}",
+                   f"problem/amp={rng.choice([1e-3, 1e-1, 1.0])}",
+                   f"problem/vflow={rng.choice([0.0, 0.5, 1.0, 3.0])}"]
+        else:
+            problem = "cpaw2d"
+            ov += [f"problem/angle={rng.choice([0.0, 26.6, 45.0, 63.4, 90.0])}",
+                   f"problem/b_perp={rng.choice([0.01, 0.1, 0.5])}",
+                   f"problem/pres={rng.choice([0.01, 0.1, 1.0])}",
+                   f"problem/v_par={rng.choice([0.0, 1.0])}"]
+        out.append((problem, ov, str(rng.choice(["roe", "hlle"]))))
+    return out
+
+
+@pytest.mark.parametrize("k", range(3))
+def test_exact_random_smooth_configurations_match_oracle(k):
+    """The same for the field loop (radius, amplitude, flow speed including
+    zero) and the circularly polarised Alfven wave (angle including the grid
+    axes, amplitude, pressure, standing or travelling)."""
+    pkg = load_pkg()
+    for problem, ov, flux in _random_smooth_configs(2000 + k, 10):
+        say(f"[gpu] random config {problem} {flux} {ov}")
+        gpu, par, tlim, _ = pkg.setup_problem(problem, ov, flux=flux, variant="exact")
+        ora, _, _ = oracle_setup(problem, ov, flux=flux)
+        compare_exact(gpu, ora, f"{problem} {flux} {ov} initial state")
+        gpu.set_limits(tlim, 8)
+        for s in range(8):
+            if ora.clock()[1] >= tlim:
+                break
+            gpu.step()
+            ora.step(tlim)
+            compare_exact(gpu, ora, f"{problem} {flux} {ov} after {s + 1} steps")
+        gpu.close()
+
+
 def test_upload_download_round_trip_and_restep():
     """athmi_upload / athmi_download stage the caller's arrays in the idle
     ping-pong buffer and convert on the device: a random padded state comes
