@@ -364,6 +364,39 @@ def test_exact_random_smooth_configurations_match_oracle(k):
         gpu.close()
 
 
+def test_batch_of_random_configurations_matches_oracle():
+    """Twelve of the random blast configurations in one batch (one tile
+    work-list, per-instance clocks, both Riemann solvers launched): every
+    instance equals its own oracle run, clock included, after 8 steps."""
+    pkg = load_pkg()
+    cfgs = _random_configs(3000, 12)
+    descs, params, oras = [], [], []
+    for ov, flux in cfgs:
+        par = pkg.read_athinput(os.path.join(INPUTS, "athinput.blast"), ov)
+        descs.append(pkg.desc_from_athinput(par, flux=flux))
+        params.append(par["problem"])
+        oras.append(oracle_setup("blast", ov, flux=flux))
+    say(f"[gpu] batch of {len(cfgs)} random blast configurations")
+    g = pkg.Grid(descs, variant="exact")
+    for k, pr in enumerate(params):
+        g.pgen_blast(k, pkg._atof(pr["radius"]), pkg._atof(pr["pamb"]), pkg._atof(pr["prat"]),
+                     pkg._atof(pr["b0"]), pkg._atof(pr["angle"]))
+    g.set_bvals()
+    g.new_dt()
+    tlim = oras[0][1]
+    g.set_limits(tlim, 8)
+    for s in range(8):
+        g.step()
+        for ora, _, _ in oras:
+            if ora.clock()[1] < tlim:
+                ora.step(tlim)
+    for k, (ora, _, _) in enumerate(oras):
+        for name, x, y in zip(FIELDS, g.download(k), ora.get()):
+            assert np.array_equal(x, y), f"instance {k} {cfgs[k]}: {name}"
+        assert g.clock(k) == ora.clock(), f"instance {k} clock"
+    g.close()
+
+
 def test_upload_download_round_trip_and_restep():
     """athmi_upload / athmi_download stage the caller's arrays in the idle
     ping-pong buffer and convert on the device: a random padded state comes
