@@ -685,9 +685,17 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_plm(AdjArgs a) {
   Prim7 W[3];
   for (int c = 0; c < 3; c++) W[c] = cons_to_prim(cons_at(kc[c]), st[qx * ps + kc[c]], eos);
   // the stencil and eigensystem sit in LDS, one per thread, and each reverse
-  // stage reads them again: the reverse then fits in registers without spills
-  __shared__ PlmCore slot[kTileX * kTileY];
-  PlmCore& P = slot[threadIdx.y * kTileX + threadIdx.x];
+  // stage reads them again: the reverse then fits in registers without spills.
+  // A slot is an odd number of doubles long, so that the 8-byte words one
+  // field occupies in consecutive lanes' slots fall in distinct LDS banks
+  // (68 doubles, an even stride, put lanes 4 or 8 to a bank)
+  struct Slot {
+    PlmCore c;
+    double pad;
+  };
+  static_assert((sizeof(Slot) / 8) % 2 == 1, "odd slot stride in doubles");
+  __shared__ Slot slot[kTileX * kTileY];
+  PlmCore& P = slot[threadIdx.y * kTileX + threadIdx.x].c;
   P.prepare(W[0], W[1], W[2], st[qx * ps + k], a.dt / (X1 ? g.dx1 : g.dx2), eos);
   double wb[3][7];
   for (int c = 0; c < 3; c++)
