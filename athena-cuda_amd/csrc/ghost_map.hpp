@@ -23,18 +23,26 @@ namespace athmi {
 // same set_bvals wrote earlier (the map follows it to where that value came
 // from), and an inner reflecting fill copies an outer ghost the outer fill has
 // not reached yet, i.e. the value it held before this set_bvals (stale).
-__device__ __forceinline__ bool ghost_src(const Inst& g, int f, int j, int i, int& sj, int& si,
-                                          bool& neg, bool& stale) {
-  const bool b1 = (f == F_B1I), b2 = (f == F_B2I);
+// The map depends on the plane only through its class: cell-centred (0), B1i
+// (1) or B2i (2). ghost_chase gives the source cell of ghost (j, i) of a class,
+// whether the x1 (nx) and x2 (ny) fills reflected it an odd number of times,
+// and whether the source is stale; false if (j, i) is not a ghost of the class.
+__device__ __forceinline__ bool ghost_chase(const Inst& g, int cls, int j, int i, int& sj,
+                                            int& si, bool& nx, bool& ny, bool& stale) {
+  const bool b1 = (cls == 1), b2 = (cls == 2);
   const int ie_x = b1 ? g.ie + 1 : g.ie;  // last non-ghost column of this plane
   const int je_y = b2 ? g.je + 1 : g.je;  // last non-ghost row
   const bool gx = (i < g.is) || (i > ie_x);
   const bool gy = (j < g.js) || (j > je_y);
-  if (!gx && !gy) return false;
-  bool nx = false, ny = false;
+  nx = false;
+  ny = false;
   stale = false;
+  if (!gx && !gy) return false;
   // the column whose value ghost column c of a non-ghost row ends with
+  // (the hops stay a loop: unrolled, the eleven planes' copies of the map
+  // made the fill kernel larger than the instruction cache)
   auto chase_x = [&](int c) -> int {
+#pragma unroll 1
     for (int hop = 0; hop < 8 && (c < g.is || c > ie_x); hop++) {
       if (c < g.is) {
         const int kk = g.is - c;
@@ -66,6 +74,7 @@ __device__ __forceinline__ bool ghost_src(const Inst& g, int f, int j, int i, in
   };
   // the same along x2 for ghost row r (whole rows, after the x1 fills)
   auto chase_y = [&](int r) -> int {
+#pragma unroll 1
     for (int hop = 0; hop < 8 && (r < g.js || r > je_y); hop++) {
       if (r < g.js) {
         const int kk = g.js - r;
@@ -104,8 +113,25 @@ __device__ __forceinline__ bool ghost_src(const Inst& g, int f, int j, int i, in
     sj = j;
     si = chase_x(i);
   }
-  neg = (f == F_M1 && nx) || (f == F_M2 && ny);
   return true;
+}
+
+__device__ __forceinline__ constexpr int ghost_class(int f) {
+  return f == F_B1I ? 1 : (f == F_B2I ? 2 : 0);
+}
+
+// reflection negates M1 across x1 and M2 across x2
+__device__ __forceinline__ constexpr bool ghost_negates(int f, bool nx, bool ny) {
+  return (f == F_M1 && nx) || (f == F_M2 && ny);
+}
+
+// the map of plane f
+__device__ __forceinline__ bool ghost_src(const Inst& g, int f, int j, int i, int& sj, int& si,
+                                          bool& neg, bool& stale) {
+  bool nx, ny;
+  const bool ghost = ghost_chase(g, ghost_class(f), j, i, sj, si, nx, ny, stale);
+  neg = ghost_negates(f, nx, ny);
+  return ghost;
 }
 
 // the map for grids of Nx1, Nx2 >= 4, where no source is stale

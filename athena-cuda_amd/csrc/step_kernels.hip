@@ -968,19 +968,33 @@ __global__ void __launch_bounds__(kGhostBlock) k_ghost(const Inst* __restrict__ 
     const double* pre = g.state[which == 1 ? clk->cur : (clk->cur ^ 1)];
     const size_t ps = g.ps;
     const uint32_t k = (uint32_t)(j * g.pitch + i);
+    // the map once per plane class (cell-centred, B1i, B2i), not per plane
+    int sj[3], si[3];
+    bool gh[3], nx[3], ny[3], stale[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      gh[c] = ghost_chase(g, c, j, i, sj[c], si[c], nx[c], ny[c], stale[c]);
+    // every plane's value is loaded before any is stored: a fill reads only
+    // non-ghost cells of the buffer it writes (the map resolves chains to
+    // them) or the other buffer, so no store can alias a later load, and the
+    // eleven loads are in flight together
+    double v[NSTATE];
 #pragma unroll
     for (int f = 0; f < NSTATE; f++) {
-      int sj, si;
-      bool neg, stale;
-      if (ghost_src(g, f, j, i, sj, si, neg, stale)) {
+      const int c = ghost_class(f);
+      v[f] = 0.0;
+      if (gh[c]) {
         if (which == 2) {
-          ST(buf + f * ps, k, LD(src + f * ps, k));
+          v[f] = LD(src + f * ps, k);
         } else {
-          const double v = LD((stale ? pre : src) + f * ps, (size_t)sj * g.pitch + si);
-          ST(buf + f * ps, k, neg ? -v : v);
+          const double x = LD((stale[c] ? pre : src) + f * ps, (size_t)sj[c] * g.pitch + si[c]);
+          v[f] = ghost_negates(f, nx[c], ny[c]) ? -x : x;
         }
       }
     }
+#pragma unroll
+    for (int f = 0; f < NSTATE; f++)
+      if (gh[ghost_class(f)]) ST(buf + f * ps, k, v[f]);
   }
 }
 
