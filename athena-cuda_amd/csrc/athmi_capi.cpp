@@ -57,11 +57,10 @@ int athmi_create(const athmi_desc* desc, int n, athmi_grid** out) {
   g->cpaw_soln.resize(n);
   g->bl.ninst = n;
   g->bl.ncu = prop.multiProcessorCount;
-  // faces marched per lane by the x2 predictor: enough 64-lane workgroups
-  // (about 2048, two per SIMD) to fill the chip, 4..16 faces each
+  // faces marched per lane by the x2 predictor (1..16)
   long cols = 0;
   for (int k = 0; k < n; k++) cols += (long)cdiv(desc[k].nx1 + 6, kX2Block) * (desc[k].nx2 + 4);
-  g->bl.shape.x2_seg = (int)std::max(4L, std::min(16L, cols / 2048));
+  g->bl.shape.x2_seg = default_x2_seg(cols);
   g->bl.shape.persist = 0;
   g->bl.shape.xcd_remap = 1;
   g->bl.shape.corr_rows = 2;  // 64 x 2 corrector tiles (at 4096^2: 1.46 against 1.60 ms for 64 x 4)
@@ -460,14 +459,14 @@ int athmi_set_shape(athmi_grid* g, int x2_seg, int persist) {
 }
 
 // Launch-shape autotuner: every candidate (tile order x x2 marching depth x
-// persistent occupancy, then the corrector's tile height) runs the same
+// persistent occupancy, then one dimension at a time) runs the same
 // `steps` main-loop steps from the same state, timed with HIP events on the
 // handle's stream through the step graph; the state and clocks are restored
 // after each candidate and the fastest shape is kept.
 int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd_remap,
                    double* ms_per_step) {
   if (!g || steps <= 0) return fail(ATHMI_EINVAL, "athmi_autotune: bad arguments");
-  static const int kSeg[] = {16, 32, 64, 128};
+  static const int kSeg[] = {2, 4, 8, 16};
   static const int kPersist[] = {0, 1, 2, 4, 8, 16, 32};
   constexpr size_t kFinalists = 6;
   constexpr int kFinalRounds = 3;
@@ -583,6 +582,10 @@ int athmi_autotune(athmi_grid* g, int steps, int* x2_seg, int* persist, int* xcd
       }
     return v;
   };
+  // the x2 marching depth between and around the main sweep's values
+  if (!rc)
+    rc = stage(variants_of({1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32}, &LaunchShape::x2_seg,
+                           best_shape.x2_seg));
   // the corrector's tile height
   if (!rc) rc = stage(variants_of({4, 2, 1}, &LaunchShape::corr_rows, best_shape.corr_rows));
   // tile order of the corrector and final kernels: whole rows or strips

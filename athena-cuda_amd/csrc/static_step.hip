@@ -167,8 +167,7 @@ int athmi_integrate_static(const athmi_desc* desc, double dt, const double* stat
   bl.ninst = 1;
   bl.ncu = ncu;
   // the handle's default launch shape for one grid
-  const long cols = (long)cdiv(d.nx1 + 6, kX2Block) * (d.nx2 + 4);
-  bl.shape.x2_seg = (int)(cols / 2048 < 4 ? 4 : (cols / 2048 > 16 ? 16 : cols / 2048));
+  bl.shape.x2_seg = default_x2_seg((long)cdiv(d.nx1 + 6, kX2Block) * (d.nx2 + 4));
   bl.shape.persist = 0;
   bl.shape.xcd_remap = 1;
   bl.shape.corr_rows = 2;

@@ -128,7 +128,8 @@ int athmi_set_stream(athmi_grid *g, void *hip_stream); /* NULL: default stream *
 int athmi_sync(athmi_grid *g);
 /* Launch shape: faces marched per x2-predictor thread, persistent
  * workgroups per CU (0 = one tile per workgroup). Picked by the autotuner;
- * the default marches 4..16 faces so that about 2048 workgroups exist. */
+ * the default marches about sqrt(C)/50 faces (1..16), C the x2 face columns
+ * of 64 lanes (1 at 256^2, 3 at 1024^2, 5 at 2048^2, 10 at 4096^2). */
 int athmi_set_shape(athmi_grid *g, int x2_seg, int persist);
 /* Tile order: 1 = each of the 8 XCDs gets one contiguous run of tiles (its L2
  * serves the neighbour rows), 0 = launch order. Results are identical. */
@@ -166,8 +167,9 @@ int athmi_get_shape(const athmi_grid *g, int *x2_seg, int *persist, int *xcd_rem
                     int *corr_rows);
 int athmi_use_graph(athmi_grid *g, int on);
 /* Autotuner: time every launch shape (x2 predictor form x x2 marching depth
- * 16..128 x persistent workgroups per CU 0..32 x tile order, then the
- * corrector's tile height 4/2/1, the corrector and final kernels' strip
+ * 2/4/8/16 x persistent workgroups per CU 0..32 x tile order, then the x2
+ * marching depth 1..32, the corrector's tile height 4/2/1, the corrector and
+ * final kernels' strip
  * width 0/2/4/8, the final tile height 4/8/16 and the fusion depth at the
  * best of those)
  * by the better of two runs of `steps` main-loop steps from the current state

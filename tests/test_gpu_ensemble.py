@@ -22,7 +22,8 @@ def assert_same(a, b, tag):
         assert np.array_equal(x, y), f"{tag} {name}: max diff {np.abs(x - y).max()}"
 
 
-@pytest.mark.parametrize("shape", [None, (64, 2, 0, 4), (16, 0, 1, 1), (32, 4, 1, 2)])
+@pytest.mark.parametrize("shape", [None, (64, 2, 0, 4), (16, 0, 1, 1), (32, 4, 1, 2), (1, 0, 1, 2),
+                                   (3, 8, 1, 1)])
 def test_batch_instances_equal_plain_runs(shape):
     pkg, E = ens()
     params = E.ensemble_params(seed=7, n=10)
@@ -100,7 +101,8 @@ def test_autotune_restores_state_and_keeps_results():
     before = [gb.download(k) for k in range(gb.n)]
     clocks = [gb.clock(k) for k in range(gb.n)]
     seg, persist, xcd, ms = gb.autotune(1)
-    assert seg in (16, 32, 64, 128) and persist in (0, 1, 2, 4, 8, 16, 32) and xcd in (0, 1) and ms > 0
+    assert seg in (1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32) and persist in (0, 1, 2, 4, 8, 16, 32)
+    assert xcd in (0, 1) and ms > 0
     shape = gb.launch_shape()  # the winner of the re-timed finalists is the shape in effect
     assert shape[:3] == (seg, persist, xcd) and shape[3] in (1, 2, 4)
     for k in range(gb.n):
