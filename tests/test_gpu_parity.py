@@ -111,6 +111,29 @@ def test_fused_corner_tile_edges(nx1, nx2, rows):
         gpu.close()
 
 
+@pytest.mark.parametrize("seg", [1, 3, 5, 7, 24])
+def test_x2_marching_depths(seg):
+    """x2 predictor segments of 1..24 faces per lane (athmi_set_shape), both
+    forms: 73 rows of cells, so 76 rows of x2 faces (js-1 .. je+2) and the
+    last segment is partial for every depth but 1; a persistent grid beside
+    one tile per workgroup."""
+    pkg = load_pkg()
+    ov = ["grid/Nx1=150", "grid/Nx2=73", "problem/radius=0.2"]
+    for kind, persist, flux in ((1, 0, "roe"), (0, 0, "hlle"), (1, 4, "hlle"), (0, 2, "roe")):
+        say(f"[gpu] x2 depth {seg} form {kind} persist {persist} blast 150x73 {flux}")
+        gpu, par, tlim, _ = pkg.setup_problem("blast", ov, flux=flux, variant="exact")
+        gpu.set_x2_kind(kind)
+        gpu.set_shape(seg, persist)
+        assert gpu.launch_shape()[:2] == (seg, persist)
+        ora, _, _ = oracle_setup("blast", ov, flux=flux)
+        gpu.set_limits(tlim, 10)
+        for s in range(10):
+            gpu.step()
+            ora.step(tlim)
+        compare_exact(gpu, ora, f"x2 depth {seg} form {kind} persist {persist} {flux} after 10 steps")
+        gpu.close()
+
+
 @pytest.mark.parametrize("strip", [2, 3, 4, 8])
 def test_strip_tile_order(strip):
     """Corrector and final tiles walked in strips of `strip` tile columns
