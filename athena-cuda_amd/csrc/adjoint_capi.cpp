@@ -8,7 +8,9 @@
 // segment's per-step slots; then each step is reversed by the adjoint
 // kernels, last step first. With two banks of slots (when they fit in device
 // memory) the re-run of segment s-1 on the handle's stream overlaps the
-// reversal of segment s on a second stream.
+// reversal of segment s on a second stream. The forward pass runs its last
+// segment through the slots itself, so the reverse starts without a re-run
+// (with one segment, K >= N, it re-runs nothing).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,6 +53,7 @@ struct athmi_adjoint {
   int nbank = 1;            // banks of segment slots (2: re-run and reversal overlap)
   int cur = 0;              // host copy of the clock's current buffer
   bool have_forward = false;
+  bool last_filled = false;  // the last segment's slots hold the forward pass's own steps
   bool det = false;         // deterministic reverse: one stream, ordered sums
   unsigned long long generation = 0;  // completed forward passes
   double* mem = nullptr;    // checkpoints | segment states | adjoint state x2 | adjoint planes
@@ -123,8 +126,10 @@ static int planes_to_dense(athmi_adjoint* h, double* dense, const double* planes
 }
 
 // re-run segment s (steps n0..n1-1) from its checkpoint into bank b on the
-// handle's stream
-static int rerun_segment(athmi_adjoint* h, int s, int b) {
+// handle's stream. finish_last: also complete the segment's last step (its
+// output state, after set_bvals, is then seg_state(b, n1 - n0)); the reverse
+// does not need it.
+static int rerun_segment(athmi_adjoint* h, int s, int b, bool finish_last = false) {
   const int n0 = s * h->every, n1 = std::min(n0 + h->every, h->nsteps);
   const size_t ps = h->g->hinst[0].ps;
   const Inst* slots = h->dslot + (size_t)(2 * b + h->cur) * h->every;
@@ -137,7 +142,7 @@ static int rerun_segment(athmi_adjoint* h, int s, int b) {
       return rc;
     HIP_TRY(hipMemcpyAsync(sc + kSlotEmf * ps, sc + S_EMF * ps, ps * sizeof(double),
                            hipMemcpyDeviceToDevice, h->g->stream));
-    if (m + 1 < n1) {
+    if (m + 1 < n1 || finish_last) {
       if ((rc = issue_on(h, ins, kSeqFinishStep, sizeof(kSeqFinishStep) / sizeof(int)))) return rc;
       h->cur ^= 1;  // k_end swaps the buffers: the output is seg[b][m + 1]
       if ((rc = issue_on(h, ins, kSeqGhostCur, 1))) return rc;
@@ -177,7 +182,7 @@ int athmi_adjoint_create(const athmi_desc* desc, double dt, int nsteps, int ever
            h->slot_doubles * nb * h->every;
   };
   size_t free_b = 0, total_b = 0;
-  h->nbank = (hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+  h->nbank = (h->nck > 1 && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
               total_for(2) * sizeof(double) < free_b / 5 * 4) ? 2 : 1;
   if (hipMalloc(&h->mem, total_for(h->nbank) * sizeof(double)) != hipSuccess ||
       hipMalloc(&h->dslot, sizeof(Inst) * 4 * h->every) != hipSuccess ||
@@ -260,17 +265,27 @@ int athmi_adjoint_set_stream(athmi_adjoint* h, void* hip_stream) {
 int athmi_adjoint_forward(athmi_adjoint* h, const double* state0, double* stateN) {
   if (!h || !state0 || !stateN) return fail(ATHMI_EINVAL, "athmi_adjoint_forward: null argument");
   int rc;
+  h->have_forward = false;
+  h->last_filled = false;
   if ((rc = dense_to_planes(h, live_state(h), state0, h->g->stream))) return rc;
   HIP_TRY(h->g->launch(&h->g->bl, SK_GHOST_CUR, h->g->stream));
-  for (int n = 0; n < h->nsteps; n++) {
+  const int slast = h->nck - 1, blast = slast % h->nbank;
+  for (int n = 0; n < slast * h->every; n++) {
     if (n % h->every == 0 &&
         (rc = copy_state(h, h->ck + (size_t)(n / h->every) * h->state_doubles, live_state(h))))
       return rc;
     if ((rc = rollout_step(h))) return rc;
   }
-  if ((rc = planes_to_dense(h, stateN, live_state(h), h->g->stream))) return rc;
+  // the last segment: its checkpoint, then its steps through the slots the
+  // reverse reads (the same kernels as a rollout step, with the primitive
+  // face states stored too)
+  if ((rc = copy_state(h, h->ck + (size_t)slast * h->state_doubles, live_state(h)))) return rc;
+  if ((rc = rerun_segment(h, slast, blast, true))) return rc;
+  const int n_last = h->nsteps - slast * h->every;
+  if ((rc = planes_to_dense(h, stateN, h->seg_state(blast, n_last), h->g->stream))) return rc;
   HIP_TRY(hipStreamSynchronize(h->g->stream));
   h->have_forward = true;
+  h->last_filled = true;
   h->generation++;
   return ATHMI_OK;
 }
@@ -314,7 +329,8 @@ int athmi_adjoint_backward(athmi_adjoint* h, const double* gradN, double* grad0)
   HIP_TRY(hipEventRecord(h->ev_join2, A));  // nothing pending on B yet
   const size_t ps = a.g.ps;
   bool rev_pending[2] = {false, false};
-  if ((rc = rerun_segment(h, h->nck - 1, (h->nck - 1) % h->nbank))) return rc;
+  const int blast = (h->nck - 1) % h->nbank;
+  if (!h->last_filled && (rc = rerun_segment(h, h->nck - 1, blast))) return rc;
   for (int s = h->nck - 1; s >= 0; s--) {
     const int b = s % h->nbank;
     const int n0 = s * h->every, n1 = std::min(n0 + h->every, h->nsteps);
@@ -354,6 +370,7 @@ int athmi_adjoint_backward(athmi_adjoint* h, const double* gradN, double* grad0)
     if (s > 0) {
       const int bn = (s - 1) % h->nbank;
       if (rev_pending[bn]) HIP_TRY(hipStreamWaitEvent(R, h->ev_rev[bn], 0));
+      if (bn == blast) h->last_filled = false;  // the last segment's slots are reused
       if ((rc = rerun_segment(h, s - 1, bn))) return rc;
     }
   }

@@ -235,6 +235,7 @@ def adjoint_leg(pkg, args):
     ratio = (t_fwd + t_bwd) / t_plain
     log(f"[bench] adjoint: plain {1e3 * t_plain:.1f} ms, forward+ckpt {1e3 * t_fwd:.1f} ms, "
         f"reverse {1e3 * t_bwd:.1f} ms, ratio {ratio:.2f} (reverse alone {t_bwd / t_plain:.2f})")
+    full = full_trajectory(A, desc, dt, N, s0, gN, g0, t_plain)
 
     def fd_check(state, tg, handle, dirs, eps=1e-6):
         """max relative |FD - adjoint| over the directions (central, eps)."""
@@ -283,7 +284,41 @@ def adjoint_leg(pkg, args):
             "fd_max_rel_density_dirs": rel_dens, "fd_density_dirs": len(dens),
             "fd_max_rel_smooth_all_fields": rel_sm, "fd_max_rel_smooth_all_fields_eps1e-8": rel_sm8,
             "fd_smooth_dirs": len(dirs),
-            "fd_smooth_steps": args.adj_smooth_steps, "grad_norm": float(g0.norm().item())}
+            "fd_smooth_steps": args.adj_smooth_steps, "grad_norm": float(g0.norm().item()),
+            "full_trajectory": full}
+
+
+def full_trajectory(A, desc, dt, N, s0, gN, g_ref, t_plain):
+    """The same rollout with one segment (checkpoint spacing N): the forward
+    pass stores every step's forward scratch (88 planes, ~0.2 GB per step at
+    512^2, ~50 GB in all) and the reverse re-runs nothing. Not the specified
+    configuration (checkpoint every 16): reported beside it. None if the
+    slots do not fit."""
+    import torch
+    try:
+        hf = A.Rollout(desc, dt, N, N)
+    except RuntimeError as e:
+        log(f"[bench] adjoint, one segment: not run ({e})")
+        return None
+    hf.forward(s0)
+    hf.backward(gN)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hf.forward(s0)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = hf.backward(gN)
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    diff = ((g - g_ref).abs().max() / g_ref.abs().max()).item()
+    hf.close()
+    ratio = (t_fwd + t_bwd) / t_plain
+    log(f"[bench] adjoint, one segment (every step stored): forward {1e3 * t_fwd:.1f} ms, "
+        f"reverse {1e3 * t_bwd:.1f} ms, ratio {ratio:.2f}; gradient vs checkpointed: "
+        f"max rel diff {diff:.1e}")
+    return {"cost_ratio": ratio, "forward_ms": 1e3 * t_fwd, "reverse_ms": 1e3 * t_bwd,
+            "checkpoint_every": N, "grad_max_rel_diff_vs_checkpointed": diff}
 
 
 def main():

@@ -20,7 +20,10 @@ extern "C" {
  * The state is held as dense device arrays [NSTATE=11][Nx2+8][Nx1+8] in the
  * order d, M1, M2, M3, E, B1c, B2c, B3c, B1i, B2i, B3i. The forward pass keeps
  * a checkpoint every `every` steps; the backward pass recomputes each segment
- * from its checkpoint and reverses it step by step. Requires ATHMI_EXACT
+ * from its checkpoint and reverses it step by step. The forward pass stores
+ * the last segment's steps as it runs them, so the backward pass re-runs all
+ * segments but that one; with every >= nsteps (one segment, each step's 99
+ * planes held: about 56 GB at 512^2, N = 256) it re-runs nothing. Requires ATHMI_EXACT
  * arithmetic (the adjoint is linearised at the branches of that forward
  * pass) and Nx1, Nx2 >= 4. Ghost values of state0 are ignored (set_bvals
  * overwrites them), so their gradient is zero. */

@@ -29,7 +29,13 @@ CASES = [
     ("hlle_blast_periodic", 24, 24, "hlle", (4, 4, 4, 4), 4, 4, True),
     # the smallest grid the adjoint takes (its ghost sources are interior)
     ("roe_smallest_mixed", 4, 5, "roe", (1, 2, 4, 4), 4, 2, False),
+    # one segment: the forward pass stores every step, the reverse re-runs none
+    ("hlle_one_segment", 20, 24, "hlle", (1, 4, 2, 4), 5, 5, True),
 ]
+# three segments over two banks of slots: the reverse re-runs segment 0 into
+# the bank that held the forward pass's last segment, so a second backward
+# pass re-runs that segment too
+DET_EXTRA = ("roe_reflect_blast_three_segments", 28, 28, "roe", (1, 1, 2, 1), 6, 2, True)
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: c[0])
@@ -240,10 +246,12 @@ def test_superseded_rollout_backward_raises_on_the_device_handle():
     h2.close()
 
 
-@pytest.mark.parametrize("case", [CASES[0], CASES[2]], ids=lambda c: c[0])
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[5], DET_EXTRA], ids=lambda c: c[0])
 def test_deterministic_reverse_is_bitwise_repeatable(case):
     """With the deterministic reverse two backward passes of one forward pass
-    give bitwise equal gradients; they agree with the default (atomic-add)
+    give bitwise equal gradients -- also where the first reads the last
+    segment's slots as the forward pass filled them and the second re-runs
+    that segment from its checkpoint; they agree with the default (atomic-add)
     reverse to roundoff and satisfy the dot-product identity against the TLM."""
     import torch
     name, n1, n2, flux, bc, nsteps, every, blast = case
