@@ -341,6 +341,39 @@ def test_exact_random_configurations_match_oracle(k):
         gpu.close()
 
 
+@pytest.mark.parametrize("k", range(8))
+def test_exact_random_configurations_random_launch_shapes(k):
+    """The random blast configurations again, each under a launch shape drawn
+    at random from everything the autotuner can pick (x2 form and marching
+    depth, persistent occupancy, tile order, corrector and final tile heights,
+    strips, fusion depth): the kernel forms combined, not one at a time."""
+    pkg = load_pkg()
+    rng = np.random.default_rng(3000 + k)
+    for ov, flux in _random_configs(2000 + k, 10):
+        shape = dict(kind=int(rng.integers(0, 2)), seg=int(rng.choice([1, 2, 3, 5, 8, 16, 24])),
+                     persist=int(rng.choice([0, 1, 4, 32])), xcd=int(rng.integers(0, 2)),
+                     rows=int(rng.choice([1, 2, 4])), final=int(rng.choice([4, 8, 16])),
+                     strip=int(rng.choice([0, 2, 3, 8])), fusion=int(rng.integers(0, 3)))
+        say(f"[gpu] random config {flux} {ov} shape {shape}")
+        gpu, par, tlim, _ = pkg.setup_problem("blast", ov, flux=flux, variant="exact")
+        gpu.set_x2_kind(shape["kind"])
+        gpu.set_shape(shape["seg"], shape["persist"])
+        gpu.set_tile_order(shape["xcd"])
+        gpu.set_corr_rows(shape["rows"])
+        gpu.set_final_rows(shape["final"])
+        gpu.set_strip(shape["strip"])
+        gpu.set_fusion(shape["fusion"])
+        ora, _, _ = oracle_setup("blast", ov, flux=flux)
+        gpu.set_limits(tlim, 8)
+        for s in range(8):
+            if ora.clock()[1] >= tlim:
+                break
+            gpu.step()
+            ora.step(tlim)
+            compare_exact(gpu, ora, f"{flux} {ov} shape {shape} after {s + 1} steps")
+        gpu.close()
+
+
 def _random_smooth_configs(seed, n):
     rng = np.random.default_rng(seed)
     out = []
