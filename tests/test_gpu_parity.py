@@ -420,20 +420,34 @@ def test_exact_random_smooth_configurations_match_oracle(k):
         gpu.close()
 
 
-def test_batch_of_random_configurations_matches_oracle():
+@pytest.mark.parametrize("k", range(4))
+def test_batch_of_random_configurations_matches_oracle(k):
     """Twelve of the random blast configurations in one batch (one tile
     work-list, per-instance clocks, both Riemann solvers launched): every
-    instance equals its own oracle run, clock included, after 8 steps."""
+    instance equals its own oracle run, clock included, after 8 steps. The
+    first batch runs at the default launch shape, the others at one drawn at
+    random from everything the autotuner can pick."""
     pkg = load_pkg()
-    cfgs = _random_configs(3000, 12)
+    cfgs = _random_configs(3000 + k, 12)
     descs, params, oras = [], [], []
     for ov, flux in cfgs:
         par = pkg.read_athinput(os.path.join(INPUTS, "athinput.blast"), ov)
         descs.append(pkg.desc_from_athinput(par, flux=flux))
         params.append(par["problem"])
         oras.append(oracle_setup("blast", ov, flux=flux))
-    say(f"[gpu] batch of {len(cfgs)} random blast configurations")
+    say(f"[gpu] batch of {len(cfgs)} random blast configurations, seed {3000 + k}")
     g = pkg.Grid(descs, variant="exact")
+    if k:
+        rng = np.random.default_rng(4000 + k)
+        g.set_x2_kind(int(rng.integers(0, 2)))
+        g.set_shape(int(rng.choice([1, 3, 8, 16, 24])), int(rng.choice([0, 2, 8, 32])))
+        g.set_tile_order(int(rng.integers(0, 2)))
+        g.set_corr_rows(int(rng.choice([1, 2, 4])))
+        g.set_final_rows(int(rng.choice([4, 8, 16])))
+        g.set_strip(int(rng.choice([0, 3, 8])))
+        g.set_fusion(int(rng.integers(0, 3)))
+        say(f"[gpu] batch shape {g.launch_shape()} form {g.x2_kind()} fusion {g.fusion()} "
+            f"strip {g.strip()} final rows {g.final_rows()}")
     for k, pr in enumerate(params):
         g.pgen_blast(k, pkg._atof(pr["radius"]), pkg._atof(pr["pamb"]), pkg._atof(pr["prat"]),
                      pkg._atof(pr["b0"]), pkg._atof(pr["angle"]))
