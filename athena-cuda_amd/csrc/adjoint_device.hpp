@@ -21,8 +21,28 @@ namespace adj {
 
 
 ATHMI_HD double sgnabs(double x) { return (x < 0.0) ? -1.0 : 1.0; }  // d|x|/dx
+
+// a / b in derivative arithmetic. The primal values a reverse recomputes
+// (they decide its branches) keep the correctly rounded division; derivative
+// values need accuracy only. On the device: the hardware reciprocal estimate
+// (relative error below 2^-24), one Newton-Raphson step r1 = r0 + r0 (1 - b r0)
+// and one residual correction of the quotient q0 + (a - b q0) r1, both with
+// fused multiply-adds: six instructions against the ten of the correctly
+// rounded sequence, and within an ulp of it (equal to it on 2^20 random
+// quotients measured; -0 / b gives +0). On the host: a / b.
+ATHMI_HD double adiv(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double r0 = __builtin_amdgcn_rcp(b);
+  const double r1 = fma(r0, fma(-b, r0, 1.0), r0);
+  const double q0 = a * r1;
+  return fma(fma(-b, q0, a), r1, q0);
+#else
+  return a / b;
+#endif
+}
+
 // adjoint of y = sqrt(x): xb += yb / (2 y), zero at y == 0
-ATHMI_HD double sqrt_b(double y, double yb) { return (y > 0.0) ? 0.5 * yb / y : 0.0; }
+ATHMI_HD double sqrt_b(double y, double yb) { return (y > 0.0) ? adiv(0.5 * yb, y) : 0.0; }
 
 ATHMI_HD double* arr(Cons7& c) { return &c.d; }
 ATHMI_HD const double* arr(const Cons7& c) { return &c.d; }
@@ -63,7 +83,7 @@ ATHMI_HD void prim_to_cons_b(const Prim7& w, double bx, const EosC& eos, const C
   wb.vx += ub.mx * w.d + ub.e * w.d * w.vx;
   wb.vy += ub.my * w.d + ub.e * w.d * w.vy;
   wb.vz += ub.mz * w.d + ub.e * w.d * w.vz;
-  wb.p += ub.e / eos.g1;
+  wb.p += adiv(ub.e, eos.g1);
   bxb += ub.e * bx;
   wb.by += ub.by + ub.e * w.by;
   wb.bz += ub.bz + ub.e * w.bz;
@@ -266,9 +286,9 @@ struct EsysPrimF {
       case 10: A.bet2b += 0.5 * X; break;
       case 38: A.bet2b -= 0.5 * X; break;
       case 12: case 40:
-        A.bet3b -= 0.5 * s / sqrtd * X; A.sqrtdb += X * 0.5 * bet3 * s / (sqrtd * sqrtd); break;
+        A.bet3b -= adiv(0.5 * s * X, sqrtd); A.sqrtdb += adiv(X * 0.5 * bet3 * s, sqrtd * sqrtd); break;
       case 13: case 41:
-        A.bet2b += 0.5 * s / sqrtd * X; A.sqrtdb -= X * 0.5 * bet2 * s / (sqrtd * sqrtd); break;
+        A.bet2b += adiv(0.5 * s * X, sqrtd); A.sqrtdb -= adiv(X * 0.5 * bet2 * s, sqrtd * sqrtd); break;
       case 15: case 29: {
         const double Y = (i == 2) ? X : -X;  // L41 = -L21
         A.nab -= Y * cs * alpha_s; A.csb -= Y * na * alpha_s; A.alpha_sb -= Y * na * cs;
@@ -281,7 +301,7 @@ struct EsysPrimF {
       case 18: case 32: A.nab += X * alpha_s * di; A.alpha_sb += X * na * di; A.dib += X * na * alpha_s; break;
       case 19: case 33: A.afpb -= X * bet2; A.bet2b -= X * afp; break;
       case 20: case 34: A.afpb -= X * bet3; A.bet3b -= X * afp; break;
-      case 25: A.asqb += X / (asq * asq); break;
+      case 25: A.asqb += adiv(X, asq * asq); break;
       default: break;
     }
   }
@@ -300,7 +320,7 @@ struct EsysPrimF {
     A.dib += A.afpb * na * af + A.aspb * na * as;
     A.qfb += A.qfnb * na;
     A.qsb += A.qsnb * na;
-    A.asqb -= A.nab * na / asq;
+    A.asqb -= adiv(A.nab * na, asq);
     // eigenvalues
     for (int k = 0; k < 7; k++) v1b += evb[k];
     A.cfb += evb[6] - evb[0];
@@ -323,22 +343,22 @@ struct EsysPrimF {
     if (abr) {
       const double Qf = (asq - cssq) / cf2_cs2, Qs = (cfsq - asq) / cf2_cs2;
       const double Qfb = sqrt_b(alpha_f, A.alpha_fb), Qsb = sqrt_b(alpha_s, A.alpha_sb);
-      A.asqb += (Qfb - Qsb) / cf2_cs2;
-      cssqb -= Qfb / cf2_cs2;
-      cfsqb += Qsb / cf2_cs2;
-      cf2_cs2b -= (Qfb * Qf + Qsb * Qs) / cf2_cs2;
+      A.asqb += adiv(Qfb - Qsb, cf2_cs2);
+      cssqb -= adiv(Qfb, cf2_cs2);
+      cfsqb += adiv(Qsb, cf2_cs2);
+      cf2_cs2b -= adiv(Qfb * Qf + Qsb * Qs, cf2_cs2);
     }
     double btsqb = 0;
     if (!btz) {
-      b2b += A.bet2b / bt;
-      b3b += A.bet3b / bt;
-      const double btb = -(A.bet2b * bet2 + A.bet3b * bet3) / bt;
+      b2b += adiv(A.bet2b, bt);
+      b3b += adiv(A.bet3b, bt);
+      const double btb = -adiv(A.bet2b * bet2 + A.bet3b * bet3, bt);
       btsqb += sqrt_b(bt, btb);
     }
     cssqb += sqrt_b(cs, A.csb);
-    A.asqb += cssqb * vaxsq / cfsq;
-    vaxsqb += cssqb * asq / cfsq;
-    cfsqb -= cssqb * cssq / cfsq;
+    A.asqb += adiv(cssqb * vaxsq, cfsq);
+    vaxsqb += adiv(cssqb * asq, cfsq);
+    cfsqb -= adiv(cssqb * cssq, cfsq);
     cfsqb += sqrt_b(cf, A.cfb);
     const double tsumb = 0.5 * cfsqb;
     cf2_cs2b += 0.5 * cfsqb;
@@ -646,8 +666,8 @@ struct PlmCore {
       double lb = dWlb[m], rb = dWrb[m];
       if (dl * dr > 0.0) {
         const double S = dl + dr;
-        lb += dWgb[m] * 2.0 * dr * dr / (S * S);
-        rb += dWgb[m] * 2.0 * dl * dl / (S * S);
+        lb += adiv(dWgb[m] * 2.0 * dr * dr, S * S);
+        rb += adiv(dWgb[m] * 2.0 * dl * dl, S * S);
       }
       wpb[m] += dWcb[m] + rb;
       wmb[m] -= dWcb[m] + lb;
@@ -738,9 +758,9 @@ struct RoeAvgF {
     const double Ql = Hl / sdl, Qr = Hr / sdr;
     const double Qb = b[4] * isd;
     isdb += b[4] * (Ql + Qr);
-    const double Hlb = Qb / sdl, Hrb = Qb / sdr;
-    sdlb -= Qb * Ql / sdl;
-    sdrb -= Qb * Qr / sdr;
+    const double Hlb = adiv(Qb, sdl), Hrb = adiv(Qb, sdr);
+    sdlb -= adiv(Qb * Ql, sdl);
+    sdrb -= adiv(Qb * Qr, sdr);
     Ulb.e += Hlb;
     Wlb.p += Hlb;
     pblb += Hlb;
@@ -753,14 +773,14 @@ struct RoeAvgF {
     Wrb.by += pbrb * Wr.by;
     Wrb.bz += pbrb * Wr.bz;
     // y = 0.5 (dl + dr) / droe
-    Wlb.d += 0.5 * b[9] / droe;
-    Wrb.d += 0.5 * b[9] / droe;
-    droeb -= b[9] * y / droe;
+    Wlb.d += adiv(0.5 * b[9], droe);
+    Wrb.d += adiv(0.5 * b[9], droe);
+    droeb -= adiv(b[9] * y, droe);
     // x = 0.5 (dby^2 + dbz^2) / S^2
     {
       const double S = sdl + sdr;
-      const double Dsqb = b[8] * 0.5 / (S * S);
-      const double Sb = -2.0 * b[8] * x / S;
+      const double Dsqb = adiv(b[8] * 0.5, S * S);
+      const double Sb = -adiv(2.0 * b[8] * x, S);
       sdlb += Sb;
       sdrb += Sb;
       const double dby = Wl.by - Wr.by, dbz = Wl.bz - Wr.bz;
@@ -1044,7 +1064,7 @@ struct EsysRoeF {
       v3b -= X * bet2;
       bet2b -= X * v3;
       vsqb += 0.5 * Rb(4, 3);
-      xb += Rb(4, 3) * g2 / g1;
+      xb += adiv(Rb(4, 3) * g2, g1);
       {
         const double r0 = Rb(4, 0), r6 = Rb(4, 6), r2 = Rb(4, 2), r4 = Rb(4, 4);
         alpha_fb += r0 * (hp - v1 * cf) + r6 * (hp + v1 * cf);
@@ -1168,9 +1188,9 @@ struct EsysRoeF {
       alsb -= X * b3;
       b3b -= X * als;
       // row 3
-      norm2b -= Lb(3, 0) * (0.5 * vsq - g2 * x / g1);
+      norm2b -= Lb(3, 0) * (0.5 * vsq - adiv(g2 * x, g1));
       vsqb -= Lb(3, 0) * norm2 * 0.5;
-      xb += Lb(3, 0) * norm2 * g2 / g1;
+      xb += adiv(Lb(3, 0) * norm2 * g2, g1);
       norm2b += Lb(3, 1) * v1 + Lb(3, 2) * v2 + Lb(3, 3) * v3 - Lb(3, 4) + Lb(3, 5) * b2 +
                 Lb(3, 6) * b3;
       v1b += Lb(3, 1) * norm2;
@@ -1184,9 +1204,9 @@ struct EsysRoeF {
       q2sb += vqstrb * v2;
       v3b += vqstrb * q3s;
       q3sb += vqstrb * v3;
-      double bet_starsqb = -(q2sb * q2s + q3sb * q3s) / bet_starsq;
-      b2sb += q2sb / bet_starsq;
-      b3sb += q3sb / bet_starsq;
+      double bet_starsqb = -adiv(q2sb * q2s + q3sb * q3s, bet_starsq);
+      b2sb += adiv(q2sb, bet_starsq);
+      b3sb += adiv(q3sb, bet_starsq);
       alpha_fb += alfb * norm1;
       alpha_sb += alsb * norm1;
       norm1b += alfb * alpha_f + alsb * alpha_s;
@@ -1207,7 +1227,7 @@ struct EsysRoeF {
       cfb += cffb * norm * alpha_f;
       alpha_sb += cssb * norm * cs;
       csb += cssb * norm * alpha_s;
-      twid_asqb -= normb * norm / twid_asq;
+      twid_asqb -= adiv(normb * norm, twid_asq);
       // ---- afpbb, aspbb, af_prime, as_prime, qf, qs
       af_pb += afpbbb * bt_star * bet_starsq;
       as_pb += aspbbb * bt_star * bet_starsq;
@@ -1228,10 +1248,10 @@ struct EsysRoeF {
         const double den = cfsq - cssq;
         const double Qf = (twid_asq - cssq) / den, Qs = (cfsq - twid_asq) / den;
         const double Qfb = sqrt_b(alpha_f, alpha_fb), Qsb = sqrt_b(alpha_s, alpha_sb);
-        twid_asqb += (Qfb - Qsb) / den;
-        cssqb -= Qfb / den;
-        cfsqb += Qsb / den;
-        const double denb = -(Qfb * Qf + Qsb * Qs) / den;
+        twid_asqb += adiv(Qfb - Qsb, den);
+        cssqb -= adiv(Qfb, den);
+        cfsqb += adiv(Qsb, den);
+        const double denb = -adiv(Qfb * Qf + Qsb * Qs, den);
         cfsqb += denb;
         cssqb -= denb;
       }
@@ -1239,14 +1259,14 @@ struct EsysRoeF {
       v3b += vbetb * bet3_star;
       b2sb += vbetb * v2 + 2.0 * bet2_star * bet_starsqb;
       b3sb += vbetb * v3 + 2.0 * bet3_star * bet_starsqb;
-      bet2b += b2sb / sgy;
-      bet3b += b3sb / sgy;
-      const double sgyb = -(b2sb * bet2_star + b3sb * bet3_star) / sgy;
+      bet2b += adiv(b2sb, sgy);
+      bet3b += adiv(b3sb, sgy);
+      const double sgyb = -adiv(b2sb * bet2_star + b3sb * bet3_star, sgy);
       gyb += sqrt_b(sgy, sgyb);
       if (!btz) {
-        b2b += bet2b / bt;
-        b3b += bet3b / bt;
-        const double btb = -(bet2b * bet2 + bet3b * bet3) / bt;
+        b2b += adiv(bet2b, bt);
+        b3b += adiv(bet3b, bt);
+        const double btb = -adiv(bet2b * bet2 + bet3b * bet3, bt);
         btsqb += sqrt_b(bt, btb);
       }
       bt_starsqb += sqrt_b(bt_star, bt_starb);
@@ -1258,9 +1278,9 @@ struct EsysRoeF {
     csb += evb[4] - evb[2];
     double vaxsqb = sqrt_b(vax, vaxb);
     cssqb += sqrt_b(cs, csb);
-    twid_asqb += cssqb * vaxsq / cfsq;
-    vaxsqb += cssqb * twid_asq / cfsq;
-    cfsqb -= cssqb * cssq / cfsq;
+    twid_asqb += adiv(cssqb * vaxsq, cfsq);
+    vaxsqb += adiv(cssqb * twid_asq, cfsq);
+    cfsqb -= adiv(cssqb * cssq, cfsq);
     cfsqb += sqrt_b(cf, cfb);
     const double tsumb = 0.5 * cfsqb;
     const double Yb = sqrt_b(cf2_cs2, 0.5 * cfsqb);
@@ -1326,11 +1346,11 @@ ATHMI_HD void hlle_speed_b(double p, double d, double Bxi, double uby, double ub
   const double asqb = 4.0 * ct2 * Yb + qsqb - tmpb;
   const double ct2b = 4.0 * asq * Yb + qsqb + tmpb;
   const double vaxsqb = qsqb + tmpb;
-  pb += asqb * eos.gamma / d;
-  db -= (asqb * asq + vaxsqb * vaxsq + ct2b * ct2) / d;
-  Bxib += vaxsqb * 2.0 * Bxi / d;
-  ubyb += ct2b * 2.0 * uby / d;
-  ubzb += ct2b * 2.0 * ubz / d;
+  pb += adiv(asqb * eos.gamma, d);
+  db -= adiv(asqb * asq + vaxsqb * vaxsq + ct2b * ct2, d);
+  Bxib += adiv(vaxsqb * 2.0 * Bxi, d);
+  ubyb += adiv(ct2b * 2.0 * uby, d);
+  ubzb += adiv(ct2b * 2.0 * ubz, d);
 }
 
 ATHMI_HD double hlle_speed(double p, double d, double Bxi, double uby, double ubz,
@@ -1427,7 +1447,7 @@ ATHMI_HD void flux_hlle_b(const Cons7& Ul, const Cons7& Ur, const Prim7& Wl, con
     tmpb += arr(Fb)[n] * (arr(Fl)[n] - arr(Fr)[n]);
   }
   const double den = (bp - bm) * (bp - bm);
-  double bpb = -tmpb * bm / den, bmb = tmpb * bp / den;
+  double bpb = -adiv(tmpb * bm, den), bmb = adiv(tmpb * bp, den);
   double pblb = 0.0, pbrb = 0.0;
   hlle_side_b(Ul, Wl, Bxi, r.pbl, bm, Flb, Ulb, Wlb, Bxib, pblb, bmb);
   hlle_side_b(Ur, Wr, Bxi, r.pbr, bp, Frb, Urb, Wrb, Bxib, pbrb, bpb);

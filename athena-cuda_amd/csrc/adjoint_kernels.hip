@@ -433,11 +433,11 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_half(AdjArgs a) {
     const double b1c = 0.5 * (c.b1h(k) + c.b1h(k + 1));
     const double b2c = 0.5 * (c.b2h(k) + c.b2h(k + pitch));
     const double emf = (b1c * m2 - b2c * m1) / dhalf;
-    b1cb = Eb * m2 / dhalf;
-    m2b = Eb * b1c / dhalf;
-    b2cb = -Eb * m1 / dhalf;
-    m1b = -Eb * b2c / dhalf;
-    dhb = -Eb * emf / dhalf;
+    b1cb = adiv(Eb * m2, dhalf);
+    m2b = adiv(Eb * b1c, dhalf);
+    b2cb = -adiv(Eb * m1, dhalf);
+    m1b = -adiv(Eb * b2c, dhalf);
+    dhb = -adiv(Eb * emf, dhalf);
     a.ain[F_D * ps + k] += dhb;
     a.ain[F_M1 * ps + k] += m1b;
     a.ain[F_M2 * ps + k] += m2b;
@@ -548,8 +548,8 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_transverse(AdjArgs a) {
     ain[F_B2C * ps + c] += db1 * X1(c, 2) + db2 * X2(c, 1);
     ain[F_B3C * ps + c] += db1 * (X1(c, 3) + x1e * v3) + db2 * (X2(c, 2) + x2e * v3);
     const double v3b = db1 * (X1(c, 6) + x1e * b3) + db2 * (X2(c, 5) + x2e * b3);
-    ain[F_M3 * ps + c] += v3b / d;
-    ain[F_D * ps + c] += -v3b * v3 / d;
+    ain[F_M3 * ps + c] += adiv(v3b, d);
+    ain[F_D * ps + c] += -adiv(v3b * v3, d);
   }
 }
 
@@ -566,11 +566,11 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_emf_cc(AdjArgs a) {
   const double d = st[F_D * ps + k], m1 = st[F_M1 * ps + k], m2 = st[F_M2 * ps + k];
   const double b1 = st[F_B1C * ps + k], b2 = st[F_B2C * ps + k];
   const double e = (b1 * m2 - b2 * m1) / d;
-  ADD(a.ain + F_B1C * ps, k, Eb * m2 / d);
-  ADD(a.ain + F_M2 * ps, k, Eb * b1 / d);
-  ADD(a.ain + F_B2C * ps, k, -Eb * m1 / d);
-  ADD(a.ain + F_M1 * ps, k, -Eb * b2 / d);
-  ADD(a.ain + F_D * ps, k, -Eb * e / d);
+  ADD(a.ain + F_B1C * ps, k, adiv(Eb * m2, d));
+  ADD(a.ain + F_M2 * ps, k, adiv(Eb * b1, d));
+  ADD(a.ain + F_B2C * ps, k, -adiv(Eb * m1, d));
+  ADD(a.ain + F_M1 * ps, k, -adiv(Eb * b2, d));
+  ADD(a.ain + F_D * ps, k, -adiv(Eb * e, d));
 }
 
 // ---------------------------------------------------------------------------
@@ -617,26 +617,26 @@ __global__ void __launch_bounds__(kTileX* kTileY) ak_pred(AdjArgs a) {
   const double dl = st[F_D * ps + k - stride], dr = st[F_D * ps + k];
   const double vl = st[mplane * ps + k - stride] / dl, vr = st[mplane * ps + k] / dr;
   const double slb = hdt * (X1 ? wlb.by : wlb.bz), srb = hdt * (X1 ? wrb.by : wrb.bz);
-  const double dblb = slb * vl / dx, vlb = slb * (b0 - bm) / dx;
-  const double dbrb = srb * vr / dx, vrb = srb * (bp - b0) / dx;
+  const double dblb = adiv(slb * vl, dx), vlb = adiv(slb * (b0 - bm), dx);
+  const double dbrb = adiv(srb * vr, dx), vrb = adiv(srb * (bp - b0), dx);
   if (a.det) {
     // stored per face; ak_plm_gather adds them to the cells in a fixed order
     double* pc = a.ab + (X1 ? A_PC1 : A_PC2) * ps;
     pc[k] = -dblb;
     pc[ps + k] = b0b + dblb - dbrb;
     pc[2 * ps + k] = dbrb;
-    pc[3 * ps + k] = vlb / dl;
-    pc[4 * ps + k] = -vlb * vl / dl;
-    pc[5 * ps + k] = vrb / dr;
-    pc[6 * ps + k] = -vrb * vr / dr;
+    pc[3 * ps + k] = adiv(vlb, dl);
+    pc[4 * ps + k] = -adiv(vlb * vl, dl);
+    pc[5 * ps + k] = adiv(vrb, dr);
+    pc[6 * ps + k] = -adiv(vrb * vr, dr);
   } else {
     ADD(a.ain + bplane * ps, k - stride, -dblb);
     ADD(a.ain + bplane * ps, k, b0b + dblb - dbrb);
     ADD(a.ain + bplane * ps, k + stride, dbrb);
-    ADD(a.ain + mplane * ps, k - stride, vlb / dl);
-    ADD(a.ain + F_D * ps, k - stride, -vlb * vl / dl);
-    ADD(a.ain + mplane * ps, k, vrb / dr);
-    ADD(a.ain + F_D * ps, k, -vrb * vr / dr);
+    ADD(a.ain + mplane * ps, k - stride, adiv(vlb, dl));
+    ADD(a.ain + F_D * ps, k - stride, -adiv(vlb * vl, dl));
+    ADD(a.ain + mplane * ps, k, adiv(vrb, dr));
+    ADD(a.ain + F_D * ps, k, -adiv(vrb * vr, dr));
   }
   st7(a.ab + (X1 ? A_WL1 : A_WL2) * ps, ps, k, wlb);
   st7(a.ab + (X1 ? A_WR1 : A_WR2) * ps, ps, k, wrb);
