@@ -428,6 +428,11 @@ ATHMI_HD void reload_fence() { asm volatile("" ::: "memory"); }
 struct PlmCore {
   double wm[7], w0[7], wp[7], bxc, dtodx;
   EsysPrimF E;
+  // quotients every stage that re-runs a part of the forward chain needs,
+  // computed once (the same expressions, so the same values): the limiter's
+  // harmonic means 2 l r / (l + r) per component, and the entries of L that
+  // divide, L[1][5] = L[5][5], L[1][6] = L[5][6] and L[3][4]
+  double gm[7], lq[3];
 
   // the stencil and the eigensystem of W[i] (Step 1)
   ATHMI_HDM void prepare(const Prim7& Wm, const Prim7& W0, const Prim7& Wp, double bxc_,
@@ -440,6 +445,20 @@ struct PlmCore {
     bxc = bxc_;
     dtodx = dtodx_;
     E.run(W0.d, W0.vx, W0.p, bxc, W0.by, W0.bz, eos.gamma);
+    for (int m = 0; m < 7; m++) {
+      const double l = w0[m] - wm[m], r = wp[m] - w0[m];
+      gm[m] = (l * r > 0.0) ? 2.0 * l * r / (l + r) : 0.0;
+    }
+    lq[0] = E.L(1, 5);
+    lq[1] = E.L(1, 6);
+    lq[2] = E.L(3, 4);
+  }
+  // entry (n, m) of L
+  ATHMI_HDM double Lq(int n, int m) const {
+    if ((n == 1 || n == 5) && m == 5) return lq[0];
+    if ((n == 1 || n == 5) && m == 6) return lq[1];
+    if (n == 3 && m == 4) return lq[2];
+    return E.L(n, m);
   }
 
   // Steps 2-4 for component n: the four characteristic projections and the
@@ -448,7 +467,7 @@ struct PlmCore {
     c = wp[m] - wm[m];
     l = w0[m] - wm[m];
     r = wp[m] - w0[m];
-    g = (l * r > 0.0) ? 2.0 * l * r / (l + r) : 0.0;
+    g = gm[m];
   }
   ATHMI_HDM void project(int n, double& ac, double& al, double& ar, double& ag) const {
     double c = 0.0, l = 0.0, r = 0.0, g = 0.0;
@@ -457,7 +476,7 @@ struct PlmCore {
       if (!prim_Lnz(n, m)) continue;
       double dc, dl, dr, dg;
       diffs(m, dc, dl, dr, dg);
-      const double L = E.L(n, m);
+      const double L = Lq(n, m);
       c = c + L * dc;
       l = l + L * dl;
       r = r + L * dr;
@@ -526,7 +545,7 @@ struct PlmCore {
         double LdW = 0.0;
 #pragma unroll
         for (int m = 0; m < 7; m++)
-          if (prim_Lnz(n, m)) LdW += E.L(n, m) * dW[m];
+          if (prim_Lnz(n, m)) LdW += Lq(n, m) * dW[m];
         if (E.ev[n] > 0.) {  // out_l += qa R[:,n], qa = c (ev6 - evn) LdW
           double qab = 0.0;
 #pragma unroll
@@ -539,7 +558,7 @@ struct PlmCore {
             if (prim_Rnz(m, n)) E.addR(m, n, out_lb[m] * qa, A);
             if (prim_Lnz(n, m)) {
               E.addL(n, m, qab * f * dW[m], A);
-              dWb[m] += qab * f * E.L(n, m);
+              dWb[m] += qab * f * Lq(n, m);
             }
           }
           evb[6] += qab * c * LdW;
@@ -561,7 +580,7 @@ struct PlmCore {
             if (prim_Rnz(m, n)) E.addR(m, n, out_rb[m] * (f * LdW) - out_lb[m] * (f2 * LdW), A);
             if (prim_Lnz(n, m)) {
               E.addL(n, m, k * dW[m], A);
-              dWb[m] += k * E.L(n, m);
+              dWb[m] += k * Lq(n, m);
             }
           }
           evb[0] += qab * c * LdW;
@@ -652,7 +671,7 @@ struct PlmCore {
         double dc, dl, dr, dg;
         diffs(m, dc, dl, dr, dg);
         E.addL(n, m, acb * dc + alb * dl + arb * dr + agb * dg, A);
-        const double L = E.L(n, m);
+        const double L = Lq(n, m);
         dWcb[m] += acb * L;
         dWlb[m] += alb * L;
         dWrb[m] += arb * L;
@@ -702,7 +721,7 @@ struct PlmF : PlmCore {
       if (E.ev[n] > 0.) {
         double qa = 0.0;
         for (int m = 0; m < 7; m++)
-          if (prim_Lnz(n, m)) qa = qa + E.L(n, m) * 0.5 * dtodx * (E.ev[6] - E.ev[n]) * dW[m];
+          if (prim_Lnz(n, m)) qa = qa + Lq(n, m) * 0.5 * dtodx * (E.ev[6] - E.ev[n]) * dW[m];
         for (int m = 0; m < 7; m++)
           if (prim_Rnz(m, n)) out_l[m] = out_l[m] + qa * E.R(m, n);
       }
@@ -711,12 +730,12 @@ struct PlmF : PlmCore {
       if (E.ev[n] < 0.) {
         double qa = 0.0;
         for (int m = 0; m < 7; m++)
-          if (prim_Lnz(n, m)) qa = qa + E.L(n, m) * 0.5 * dtodx * (E.ev[0] - E.ev[n]) * dW[m];
+          if (prim_Lnz(n, m)) qa = qa + Lq(n, m) * 0.5 * dtodx * (E.ev[0] - E.ev[n]) * dW[m];
         for (int m = 0; m < 7; m++)
           if (prim_Rnz(m, n)) out_r[m] = out_r[m] + qa * E.R(m, n);
         qa = 0.0;
         for (int m = 0; m < 7; m++)
-          if (prim_Lnz(n, m)) qa = qa + E.L(n, m) * 0.5 * dtodx * (E.ev[n] - E.ev[6]) * dW[m];
+          if (prim_Lnz(n, m)) qa = qa + Lq(n, m) * 0.5 * dtodx * (E.ev[n] - E.ev[6]) * dW[m];
         for (int m = 0; m < 7; m++)
           if (prim_Rnz(m, n)) out_l[m] = out_l[m] - qa * E.R(m, n);
       }
