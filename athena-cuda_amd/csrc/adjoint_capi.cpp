@@ -27,9 +27,16 @@ namespace {
 // forward scratch of one step: predictor, predictor corner emf, corrector fluxes
 const int kSeqForwardScratch[] = {SK_BEGIN_ONCE, SK_PRED_X1_SAVE, SK_PRED_X2_SAVE, SK_CORNER_P, SK_CORR,
                                   SK_CORR2};
-// the rest of integrate_2d after kSeqForwardScratch
-const int kSeqFinishStep[] = {SK_CORNER_C, SK_FINAL, SK_GHOST_CARRY, SK_END_ONCE};
+// the rest of integrate_2d after kSeqForwardScratch, and the set_bvals that
+// follows it. integrate_2d leaves the ghost cells as they were (the drop-in's
+// carry copies them into the new buffer), but the set_bvals right after it
+// writes every ghost cell from non-ghost ones when Nx1, Nx2 >= 4, which the
+// adjoint requires: its steps skip the carry
+const int kSeqFinishStep[] = {SK_CORNER_C, SK_FINAL, SK_END_ONCE};
 const int kSeqGhostCur[] = {SK_GHOST_CUR};
+// a whole step so: integrate_2d without the carry (then set_bvals)
+const int kSeqRolloutStep[] = {SK_BEGIN_ONCE, SK_PRED_X1, SK_PRED_X2, SK_CORNER_P, SK_CORR,
+                               SK_CORR2,      SK_CORNER_C, SK_FINAL,  SK_END_ONCE};
 // the reverse of one step on two streams: the x1 and x2 halves of Steps 8
 // and 1-2 are independent (their adjoints meet in ain through atomic adds)
 const int kRevHead[] = {AK_FOLD_OUT, AK_ZERO, AK_UPDATE, AK_CORNER_C};  // then fork
@@ -92,7 +99,7 @@ static int adj_issue(athmi_adjoint* h, const int* seq, size_t n) {
 
 // one step of the rollout: integrate_2d with the clock's dt, then set_bvals
 static int rollout_step(athmi_adjoint* h) {
-  int rc = adj_issue(h, kSeqIntegrate, sizeof(kSeqIntegrate) / sizeof(int));
+  int rc = adj_issue(h, kSeqRolloutStep, sizeof(kSeqRolloutStep) / sizeof(int));
   if (rc) return rc;
   h->cur ^= 1;  // k_end swaps the buffers
   HIP_TRY(h->g->launch(&h->g->bl, SK_GHOST_CUR, h->g->stream));
