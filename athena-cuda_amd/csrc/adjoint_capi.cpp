@@ -44,9 +44,10 @@ const int kRevMainA[] = {AK_RIEMANN_C1, AK_HALF};                       // || AK
 const int kRevMid[] = {AK_TRANSVERSE, AK_CORNER_P, AK_EMF_CC};          // after the join
 const int kRevTailA[] = {AK_PRED_X1, AK_PLM_X1, AK_PLMG_X1};            // || x2 tail
 const int kRevTailB[] = {AK_PRED_X2, AK_PLM_X2, AK_PLMG_X2};
-// a slot of the segment's forward scratch: the step's NSCRATCH planes, its
-// predictor corner emf (the corrector's corner kernel overwrites S_EMF) and
-// the predictors' NPRIMS primitive face states
+// a slot of the segment's forward scratch: the step's NSCRATCH planes, a copy
+// of its predictor corner emf (used at fusion depth 0, where the corrector's
+// corner kernel overwrites S_EMF) and the predictors' NPRIMS primitive face
+// states
 constexpr int kSlotEmf = NSCRATCH;
 constexpr int kSlotPrims = NSCRATCH + 1;
 constexpr int kSlotPlanes = NSCRATCH + 1 + NPRIMS;
@@ -147,8 +148,12 @@ static int rerun_segment(athmi_adjoint* h, int s, int b, bool finish_last = fals
     double* sc = h->slot(b, m - n0);
     if ((rc = issue_on(h, ins, kSeqForwardScratch, sizeof(kSeqForwardScratch) / sizeof(int))))
       return rc;
-    HIP_TRY(hipMemcpyAsync(sc + kSlotEmf * ps, sc + S_EMF * ps, ps * sizeof(double),
-                           hipMemcpyDeviceToDevice, h->g->stream));
+    // at fusion depth 0 the corrector's corner kernel overwrites the
+    // predictor corners; otherwise they stay in S_EMF and the reverse reads
+    // them there
+    if (!h->g->bl.shape.fuse_corner)
+      HIP_TRY(hipMemcpyAsync(sc + kSlotEmf * ps, sc + S_EMF * ps, ps * sizeof(double),
+                             hipMemcpyDeviceToDevice, h->g->stream));
     if (m + 1 < n1 || finish_last) {
       if ((rc = issue_on(h, ins, kSeqFinishStep, sizeof(kSeqFinishStep) / sizeof(int)))) return rc;
       h->cur ^= 1;  // k_end swaps the buffers: the output is seg[b][m + 1]
@@ -348,7 +353,7 @@ int athmi_adjoint_backward(athmi_adjoint* h, const double* gradN, double* grad0)
       const double* sc = h->slot(b, m - n0);
       a.st = h->seg_state(b, m - n0);
       a.sc = sc;
-      a.emfp = sc + kSlotEmf * ps;
+      a.emfp = sc + (h->g->bl.shape.fuse_corner ? (int)S_EMF : kSlotEmf) * ps;
       a.pw = sc + kSlotPrims * ps;
       a.ain = ain;
       a.aout = aout;
