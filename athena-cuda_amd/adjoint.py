@@ -18,7 +18,6 @@ under the same soname as the one the library links, and whichever is loaded
 first serves the whole process.
 """
 import ctypes
-import math
 
 import numpy as np
 import torch
@@ -42,8 +41,9 @@ def _bind():
         L.athmi_adjoint_backward.argtypes = [_P, _P, _P]
         L.athmi_adjoint_generation.argtypes = [_P, ctypes.POINTER(ctypes.c_ulonglong)]
         L.athmi_adjoint_set_deterministic.argtypes = [_P, _I]
+        L.athmi_adjoint_every.argtypes = [_P, ctypes.POINTER(_I)]
         for n in ("create", "destroy", "set_stream", "forward", "backward", "generation",
-                  "set_deterministic"):
+                  "set_deterministic", "every"):
             getattr(L, "athmi_adjoint_" + n).restype = _I
         L._adjoint_bound = True
     return L
@@ -53,8 +53,11 @@ class Rollout:
     """Device handle of one fixed-dt rollout with sqrt(N) checkpointing."""
 
     def __init__(self, desc, dt, nsteps, every=None, deterministic=False):
+        """every: checkpoint spacing; None lets the library choose (one
+        segment, every step stored and none re-run, when that fits in the
+        free device memory, else sqrt(nsteps))."""
         L = _bind()
-        every = every or max(1, int(round(math.sqrt(nsteps))))
+        every = int(every) if every else 0
         bc = desc.get("bc", (4, 4, 4, 4))
         flux = desc.get("flux", "roe")
         d = _Desc(int(desc["nx1"]), int(desc["nx2"]), desc["x1min"], desc["x1max"],
@@ -69,7 +72,9 @@ class Rollout:
             _check(L.athmi_adjoint_set_deterministic(h, 1), "athmi_adjoint_set_deterministic")
         self.deterministic = bool(deterministic)
         self.desc = dict(desc)
-        self.dt, self.nsteps, self.every = dt, nsteps, every
+        k = ctypes.c_int()
+        _check(L.athmi_adjoint_every(h, ctypes.byref(k)), "athmi_adjoint_every")
+        self.dt, self.nsteps, self.every = dt, nsteps, k.value
         self.shape = (11, int(desc["nx2"]) + 8, int(desc["nx1"]) + 8)
 
     def close(self):

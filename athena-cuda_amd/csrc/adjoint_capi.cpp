@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -167,7 +168,7 @@ extern "C" {
 
 int athmi_adjoint_create(const athmi_desc* desc, double dt, int nsteps, int every,
                          athmi_adjoint** out) {
-  if (!desc || !out || nsteps <= 0 || every <= 0 || !(dt > 0.0))
+  if (!desc || !out || nsteps <= 0 || !(dt > 0.0))
     return fail(ATHMI_EINVAL, "athmi_adjoint_create: bad arguments");
   if (desc->variant != ATHMI_EXACT)
     return fail(ATHMI_EINVAL,
@@ -183,19 +184,28 @@ int athmi_adjoint_create(const athmi_desc* desc, double dt, int nsteps, int ever
   }
   h->dt = dt;
   h->nsteps = nsteps;
-  h->every = std::min(every, nsteps);
-  h->nck = (nsteps + h->every - 1) / h->every;
   const Inst& I = h->g->hinst[0];
   h->state_doubles = (size_t)NSTATE * I.ps;
   h->slot_doubles = (size_t)kSlotPlanes * I.ps;
+  size_t free_b = 0, total_b = 0;
+  const bool have_free = hipMemGetInfo(&free_b, &total_b) == hipSuccess;
+  if (every <= 0) {
+    // automatic: one segment (every step stored, no re-run) when its slots
+    // leave a fifth of the free memory, else sqrt(N)
+    const size_t one = h->state_doubles * (1 + (size_t)(nsteps + 1) + 2) + (size_t)NADJ * I.ps +
+                       h->slot_doubles * (size_t)nsteps;
+    every = (have_free && one * sizeof(double) < free_b / 5 * 4)
+                ? nsteps
+                : std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
+  }
+  h->every = std::min(every, nsteps);
+  h->nck = (nsteps + h->every - 1) / h->every;
   // two banks of segment slots when they leave a fifth of the free memory
   auto total_for = [&](int nb) {
     return h->state_doubles * (h->nck + (size_t)nb * (h->every + 1) + 2) + (size_t)NADJ * I.ps +
            h->slot_doubles * nb * h->every;
   };
-  size_t free_b = 0, total_b = 0;
-  h->nbank = (h->nck > 1 && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
-              total_for(2) * sizeof(double) < free_b / 5 * 4) ? 2 : 1;
+  h->nbank = (h->nck > 1 && have_free && total_for(2) * sizeof(double) < free_b / 5 * 4) ? 2 : 1;
   if (hipMalloc(&h->mem, total_for(h->nbank) * sizeof(double)) != hipSuccess ||
       hipMalloc(&h->dslot, sizeof(Inst) * 4 * h->every) != hipSuccess ||
       hipStreamCreateWithFlags(&h->astream, hipStreamNonBlocking) != hipSuccess) {
@@ -299,6 +309,12 @@ int athmi_adjoint_forward(athmi_adjoint* h, const double* state0, double* stateN
   h->have_forward = true;
   h->last_filled = true;
   h->generation++;
+  return ATHMI_OK;
+}
+
+int athmi_adjoint_every(const athmi_adjoint* h, int* every) {
+  if (!h || !every) return fail(ATHMI_EINVAL, "athmi_adjoint_every: null argument");
+  *every = h->every;
   return ATHMI_OK;
 }
 

@@ -23,7 +23,10 @@ extern "C" {
  * from its checkpoint and reverses it step by step. The forward pass stores
  * the last segment's steps as it runs them, so the backward pass re-runs all
  * segments but that one; with every >= nsteps (one segment, each step's 99
- * planes held: about 56 GB at 512^2, N = 256) it re-runs nothing. Requires ATHMI_EXACT
+ * planes held: about 56 GB at 512^2, N = 256) it re-runs nothing. every <= 0
+ * chooses: one segment when its slots leave a fifth of the free device
+ * memory, else round(sqrt(nsteps)) (athmi_adjoint_every gives the choice).
+ * Requires ATHMI_EXACT
  * arithmetic (the adjoint is linearised at the branches of that forward
  * pass) and Nx1, Nx2 >= 4. Ghost values of state0 are ignored (set_bvals
  * overwrites them), so their gradient is zero. */
@@ -31,6 +34,9 @@ typedef struct athmi_adjoint athmi_adjoint;
 int athmi_adjoint_create(const athmi_desc *desc, double dt, int nsteps, int every,
                          athmi_adjoint **out);
 int athmi_adjoint_destroy(athmi_adjoint *h);
+/* the checkpoint spacing in use (every <= 0 at create: the automatic choice);
+ * no reference counterpart */
+int athmi_adjoint_every(const athmi_adjoint *h, int *every);
 int athmi_adjoint_set_stream(athmi_adjoint *h, void *hip_stream);
 /* state0, stateN: device pointers (dense layout above). Both calls run on the
  * handle's stream and synchronise it before returning. */

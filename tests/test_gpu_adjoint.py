@@ -282,3 +282,30 @@ def test_deterministic_reverse_is_bitwise_repeatable(case):
     assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
     hd.close()
     h.close()
+
+
+def test_automatic_checkpoint_spacing_keeps_every_step_when_it_fits():
+    """every=None: the library keeps every step (one segment, no re-run) when
+    that fits in the free device memory, as it does for a small grid, and the
+    gradient equals the checkpointed one (every 2) to roundoff."""
+    import torch
+    name, n1, n2, flux, bc, nsteps, every, blast = CASES[1]
+    print(f"[gpu] automatic checkpoint spacing {name} {n1}x{n2} N={nsteps}", flush=True)
+    A = adj_mod()
+    s0 = rollout_state(zlib.crc32(name.encode()) % 1000, n1, n2, blast=blast)
+    dt = stable_dt(s0, n1, n2)
+    desc = dict(nx1=n1, nx2=n2, x1min=0.0, x1max=1.0, x2min=0.0, x2max=1.0, gamma=GAMMA,
+                bc=bc, flux=flux)
+    dev = torch.device("cuda:0")
+    yb = torch.from_numpy(np.random.default_rng(3).normal(size=s0.shape)).to(dev)
+    ha = A.Rollout(desc, dt, nsteps)
+    assert ha.every == nsteps
+    hk = A.Rollout(desc, dt, nsteps, 2)
+    assert hk.every == 2
+    sa = ha.forward(torch.from_numpy(s0).to(dev))
+    sk = hk.forward(torch.from_numpy(s0).to(dev))
+    assert torch.equal(sa, sk)
+    ga, gk = ha.backward(yb), hk.backward(yb)
+    assert (ga - gk).abs().max().item() <= 1e-12 * gk.abs().max().item()
+    ha.close()
+    hk.close()
