@@ -39,8 +39,13 @@ def build(verbose_resources=False):
             # the scheduler's register-pressure trackers: the two-wave x1
             # predictor spills less (1.85 against 2.13 ms at 4096^2; the whole
             # step 3% faster)
+            # the fast variant also lets division and square root use the
+            # reciprocal estimate refined by Newton steps (about an ulp, not
+            # correctly rounded): it is held to a tolerance, not to the
+            # reference's values
+            approx = ["-freciprocal-math", "-fapprox-func"] if variant == "fast" else []
             cmd = [HIPCC, ARCH, *COMMON, *NO_PK, f"-DATHMI_VARIANT={variant}",
-                   f"-ffp-contract={contract}", "-mllvm", "-amdgpu-use-amdgpu-trackers",
+                   f"-ffp-contract={contract}", *approx, "-mllvm", "-amdgpu-use-amdgpu-trackers",
                    "-c", os.path.join(CSRC, "step_kernels.hip"), "-o", o]
             if verbose_resources:
                 cmd.append("-Rpass-analysis=kernel-resource-usage")

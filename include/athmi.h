@@ -38,7 +38,9 @@ extern "C" {
 #define ATHMI_FLUX_HLLE 1 /* flux_hlle.c:60 */
 
 #define ATHMI_EXACT 0 /* one rounding per operation (no FMA contraction) */
-#define ATHMI_FAST 1  /* FMA-contracted arithmetic */
+#define ATHMI_FAST 1  /* FMA-contracted arithmetic; division and square root from
+                         the reciprocal estimates refined by Newton steps
+                         (about an ulp) */
 
 typedef struct athmi_grid athmi_grid;
 
